@@ -1,0 +1,256 @@
+"""ctypes binding of libeqfluid.so (include/eqfluid.h).
+
+Loads the in-tree library built by ``equilibrium_amd.build`` and raises if it
+is missing: there is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+LIB_PATH = os.path.join(PKG_DIR, "libeqfluid.so")
+HEADER_PATH = os.path.join(REPO_DIR, "include", "eqfluid.h")
+
+EQF_OK, EQF_EINVAL, EQF_ENOMEM, EQF_EDEVICE, EQF_EBUSY, EQF_ENOTFOUND = range(6)
+DENSITY, SCRATCH, VX, VY, VX0, VY0 = range(6)
+DEFAULT_WALL, NO_WALL = 0, 1
+GS_WAVEFRONT, GS_REDBLACK = 0, 1
+
+
+class EqfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"eqf error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_uint32),
+        ("delta_t", ctypes.c_float),
+        ("frames", ctypes.c_int64),
+        ("diffusion", ctypes.c_float),
+        ("viscosity", ctypes.c_float),
+        ("gs_iters", ctypes.c_int64),
+        ("pressure_iters", ctypes.c_int64),
+        ("gs_mode", ctypes.c_int32),
+        ("init_times", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def declared_functions():
+    """Names of every function declared in include/eqfluid.h."""
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(eqf_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        PP = ctypes.POINTER(ctypes.c_void_p)
+        I = ctypes.c_int
+        I64 = ctypes.c_int64
+        U32 = ctypes.c_uint32
+        F = ctypes.c_float
+        sig = {
+            "eqf_params_default": (None, [ctypes.POINTER(Params)]),
+            "eqf_last_error": (ctypes.c_char_p, []),
+            "eqf_ctx_create": (I, [I, PP]),
+            "eqf_ctx_destroy": (I, [P]),
+            "eqf_scene_create": (I, [P, ctypes.POINTER(Params), PP]),
+            "eqf_scene_clone": (I, [P, PP]),
+            "eqf_scene_destroy": (I, [P]),
+            "eqf_scene_get_params": (I, [P, ctypes.POINTER(Params)]),
+            "eqf_scene_set_params": (I, [P, ctypes.POINTER(Params)]),
+            "eqf_rect_validate": (I, [I64, I64, I64, I64, U32]),
+            "eqf_fill_rect": (I, [P, I64, I64, I64, I64]),
+            "eqf_set_mask": (I, [P, P]),
+            "eqf_get_mask": (I, [P, P]),
+            "eqf_add_density": (I, [P, U32, U32, F]),
+            "eqf_add_velocity": (I, [P, U32, U32, F, F]),
+            "eqf_add_source": (I, [P, I, P]),
+            "eqf_set_frame_source": (I, [P, I, P]),
+            "eqf_upload": (I, [P, I, P]),
+            "eqf_download": (I, [P, I, P]),
+            "eqf_step": (I, [P, I64]),
+            "eqf_sync": (I, [P]),
+            "eqf_batch_create": (I, [P, PP, I, PP]),
+            "eqf_batch_step": (I, [P, I64]),
+            "eqf_batch_sync": (I, [P]),
+            "eqf_batch_destroy": (I, [P]),
+            "eqf_batch_last_ms": (I, [P, ctypes.POINTER(ctypes.c_double)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().eqf_last_error().decode()
+
+
+def check(rc):
+    if rc != EQF_OK:
+        raise EqfError(rc, last_error())
+    return rc
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def default_params():
+    p = Params()
+    lib().eqf_params_default(ctypes.byref(p))
+    return p
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().eqf_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().eqf_ctx_destroy(self.h)
+            self.h = None
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Scene:
+    """One device-resident `Fluid` state (thin handle over eqf_scene)."""
+
+    def __init__(self, params, ctx=None, _handle=None):
+        self.ctx = ctx or default_context()
+        if _handle is not None:
+            self.h = _handle
+        else:
+            h = ctypes.c_void_p()
+            check(lib().eqf_scene_create(self.ctx.h, ctypes.byref(params), ctypes.byref(h)))
+            self.h = h
+        p = Params()
+        check(lib().eqf_scene_get_params(self.h, ctypes.byref(p)))
+        self.params = p
+        self.size = p.size
+
+    def clone(self):
+        h = ctypes.c_void_p()
+        check(lib().eqf_scene_clone(self.h, ctypes.byref(h)))
+        return Scene(None, self.ctx, _handle=h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().eqf_scene_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        check(lib().eqf_scene_set_params(self.h, ctypes.byref(p)))
+        self.params = p
+
+    def fill_rect(self, x0, y0, x1, y1):
+        check(lib().eqf_fill_rect(self.h, x0, y0, x1, y1))
+
+    def set_mask(self, mask):
+        m = np.ascontiguousarray(mask, np.uint8)
+        check(lib().eqf_set_mask(self.h, ptr(m)))
+
+    def get_mask(self):
+        m = np.empty(self.size * self.size, np.uint8)
+        check(lib().eqf_get_mask(self.h, ptr(m)))
+        return m
+
+    def add_density(self, x, y, amount):
+        check(lib().eqf_add_density(self.h, x, y, amount))
+
+    def add_velocity(self, x, y, ax, ay):
+        check(lib().eqf_add_velocity(self.h, x, y, ax, ay))
+
+    def add_source(self, field, amounts):
+        a = np.ascontiguousarray(amounts, np.float32).ravel()
+        check(lib().eqf_add_source(self.h, field, ptr(a)))
+
+    def set_frame_source(self, field, amounts):
+        a = None if amounts is None else np.ascontiguousarray(amounts, np.float32).ravel()
+        check(lib().eqf_set_frame_source(self.h, field, ptr(a)))
+
+    def upload(self, field, arr):
+        a = np.ascontiguousarray(arr, np.float32).ravel()
+        assert a.size == self.size * self.size
+        check(lib().eqf_upload(self.h, field, ptr(a)))
+
+    def download(self, field):
+        a = np.empty(self.size * self.size, np.float32)
+        check(lib().eqf_download(self.h, field, ptr(a)))
+        return a
+
+    def step(self, nframes=1):
+        check(lib().eqf_step(self.h, nframes))
+
+    def sync(self):
+        check(lib().eqf_sync(self.h))
+
+
+class Batch:
+    def __init__(self, scenes, ctx=None):
+        self.ctx = ctx or scenes[0].ctx
+        self.scenes = list(scenes)
+        arr = (ctypes.c_void_p * len(scenes))(*[s.h for s in scenes])
+        h = ctypes.c_void_p()
+        check(lib().eqf_batch_create(self.ctx.h, arr, len(scenes), ctypes.byref(h)))
+        self.h = h
+
+    def step(self, nframes=1):
+        check(lib().eqf_batch_step(self.h, nframes))
+
+    def sync(self):
+        check(lib().eqf_batch_sync(self.h))
+
+    def last_ms(self):
+        ms = ctypes.c_double()
+        check(lib().eqf_batch_last_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().eqf_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

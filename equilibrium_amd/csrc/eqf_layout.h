@@ -1,0 +1,96 @@
+// eqf_layout.h -- scene layout shared by the host runtime and the kernels.
+//
+// A scene of `size` S = N + 2 (N interior rows/columns plus the 1-cell wall
+// ring of the reference, fluid.rs:552-570) keeps its six fields resident in
+// HBM.  Two layouts exist:
+//
+//  * CYCLIC (exact wavefront mode).  Interior rows are grouped in strips of 64
+//    rows (strip s = rows 64s+1 .. 64s+64, lane r = row-in-strip).  Cell
+//    (c, j), 1 <= c, j <= N, with s = (j-1)/64, r = (j-1)%64, is stored at
+//    slot t = (c-1+r) mod N of strip s:  offset (s*N + t)*64 + r.  A wave that
+//    owns a strip walks the Gauss-Seidel anti-diagonal wavefront (lane r at
+//    column t-r+1 at step t) and every step touches one 256-byte slot row of
+//    each operand: fully coalesced, no LDS staging.  The ring follows the
+//    interior: top[c] (row 0), bottom[c] (row S-1), left[j] (column 0),
+//    right[j] (column S-1), 4*S floats; corners live in top[0], top[S-1],
+//    bottom[0], bottom[S-1].
+//
+//  * ROWMAJOR (red-black mode).  Plain `x + y*S` like the reference
+//    (fluid.rs:31-35), ring included.
+//
+// Field order follows the Fluid struct (fluid.rs:59-77).
+#pragma once
+#include <stdint.h>
+
+#ifdef __HIPCC__
+#define EQF_HD __host__ __device__
+#else
+#define EQF_HD
+#endif
+
+namespace eqf {
+
+enum Field { F_D = 0, F_SC = 1, F_U = 2, F_V = 3, F_U0 = 4, F_V0 = 5, NFIELDS = 6 };
+enum Mode { M_ROW = 0, M_COL = 1, M_PASSIVE = 2 };  // Orientation, fluid.rs:22-29
+enum Layout { L_CYCLIC = 0, L_ROWMAJOR = 1 };
+
+// Per-cell boundary code (one byte per interior cell, laid out like a field).
+enum CodeBits : uint32_t {
+  C_FLUID = 1u,     // cell is ContainerWall::NoWall
+  C_LWALL = 2u,     // (c-1, j) is a wall
+  C_RWALL = 4u,     // (c+1, j) is a wall
+  C_DWALL = 8u,     // (c, j+1) is a wall ("down" in fluid.rs:146)
+  C_UWALL = 16u,    // (c, j-1) is a wall ("up" in fluid.rs:145)
+  C_FIXUP = 32u,    // cyclic only: (c, j-1) is fluid, sits in the strip above,
+                    // (c, j) is a wall and (c, j-2) is not: the wave that
+                    // computes (c, j) writes the AdjustColumn value of (c, j-1)
+  C_COLF = 64u,     // column c holds at least one NoWall cell (Passive ring copy)
+};
+
+struct SceneDesc {
+  float* fld;            // NFIELDS fields, `fstride` floats apart
+  const float* src;      // per-frame sources (density, vx, vy) or nullptr
+  const uint8_t* code;   // boundary codes, interior cells only (same indexing as fields)
+  const uint8_t* rowf;   // [S] row j holds a NoWall cell
+  const int2* bnd;       // (dst, src) field offsets: ROW list, COL list, PASSIVE list
+  float* hraw;           // cyclic: ns*N raw last-row values handed to the strip below
+  int64_t fstride;       // floats per field
+  int S, N, ns;          // size, interior size, strips
+  int nbnd[3];           // list lengths per mode
+  int diff_iters, press_iters;
+  float a_visc, cr_visc; // diffuse coefficients for velocity (fluid.rs:286-294, 311)
+  float a_diff, cr_diff; // diffuse coefficients for density
+  float dt0;             // dt * (S-2)         (fluid.rs:390)
+  float sf;              // (float)S           (fluid.rs:345, 396)
+  float sfm1;            // (float)S - 1.0f    (fluid.rs:403)
+  int has_src;
+  int pad_;
+};
+
+// Ring offset inside a cyclic field.
+EQF_HD inline int64_t cyc_ring(int N, int ns) { return (int64_t)ns * N * 64; }
+
+// Offset of cell (x, y), 0 <= x, y <= S-1, inside one field.
+EQF_HD inline int64_t cyc_off(int x, int y, int S, int N, int ns) {
+  const int64_t ro = cyc_ring(N, ns);
+  if (y == 0) return ro + x;
+  if (y == S - 1) return ro + S + x;
+  if (x == 0) return ro + 2 * S + y;
+  if (x == S - 1) return ro + 3 * S + y;
+  const int j = y - 1, s = j >> 6, r = j & 63;
+  int t = x - 1 + r;
+  if (t >= N) t -= N;
+  return ((int64_t)s * N + t) * 64 + r;
+}
+
+EQF_HD inline int64_t cyc_field_floats(int S, int N, int ns) {
+  int64_t n = cyc_ring(N, ns) + 4 * (int64_t)S;
+  return (n + 63) & ~(int64_t)63;
+}
+
+EQF_HD inline int64_t rm_field_floats(int S) {
+  int64_t n = (int64_t)S * S;
+  return (n + 63) & ~(int64_t)63;
+}
+
+}  // namespace eqf

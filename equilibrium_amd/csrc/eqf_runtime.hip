@@ -1,0 +1,720 @@
+// eqf_runtime.hip -- host side of libeqfluid.so: the C ABI of include/eqfluid.h.
+//
+// Scene state lives on the device.  The host keeps only the parameters and the
+// obstacle mask (static during a run, fluid.rs:610-619), from which it derives
+// the per-cell boundary codes and the set_boundaries lists the kernels use.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "eqf_internal.h"
+
+using namespace eqf;
+
+template <int D, int B, int MAXT>
+__global__ void eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes);
+
+
+// ----------------------------------------------------------------- errors --
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(EQF_EDEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char* eqf_last_error(void) { return g_err.c_str(); }
+
+extern "C" void eqf_params_default(eqf_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->size = 128;       // configs.rs:19
+  p->delta_t = 0.02f;  // configs.rs:17
+  p->frames = 16;      // configs.rs:18
+  p->diffusion = 0.0f; // configs.rs:53
+  p->viscosity = 0.001f;  // configs.rs:54
+  p->gs_iters = -1;
+  p->pressure_iters = -1;
+  p->gs_mode = EQF_GS_WAVEFRONT;
+  p->init_times = 1;
+}
+
+static int64_t resolved_gs(const eqf_params& p) { return p.gs_iters < 0 ? p.frames : p.gs_iters; }
+static int64_t resolved_press(const eqf_params& p) {
+  return p.pressure_iters < 0 ? resolved_gs(p) : p.pressure_iters;
+}
+
+int64_t eqf_scene_off(const eqf_scene* s, int x, int y) {
+  return s->layout == L_CYCLIC ? cyc_off(x, y, s->S, s->N, s->ns) : (int64_t)x + (int64_t)y * s->S;
+}
+
+static float* field_ptr(const eqf_scene* s, int f) { return s->d_fld + (int64_t)f * s->fstride; }
+
+// row-major host array -> device field
+static int upload_rm(eqf_scene* s, float* dfield, const float* host) {
+  const int S = s->S;
+  if (s->layout == L_ROWMAJOR) {
+    HIPCHK(hipMemcpyAsync(dfield, host, sizeof(float) * (size_t)S * S, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EQF_OK;
+  }
+  s->hbuf.assign(s->fstride, 0.0f);
+  for (int y = 0; y < S; ++y)
+    for (int x = 0; x < S; ++x) s->hbuf[eqf_scene_off(s, x, y)] = host[(size_t)x + (size_t)y * S];
+  HIPCHK(hipMemcpyAsync(dfield, s->hbuf.data(), sizeof(float) * s->fstride, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return EQF_OK;
+}
+
+static int download_rm(const eqf_scene* cs, const float* dfield, float* host) {
+  eqf_scene* s = const_cast<eqf_scene*>(cs);
+  const int S = s->S;
+  if (s->layout == L_ROWMAJOR) {
+    HIPCHK(hipMemcpyAsync(host, dfield, sizeof(float) * (size_t)S * S, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EQF_OK;
+  }
+  s->hbuf.resize(s->fstride);
+  HIPCHK(hipMemcpyAsync(s->hbuf.data(), dfield, sizeof(float) * s->fstride, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int y = 0; y < S; ++y)
+    for (int x = 0; x < S; ++x) host[(size_t)x + (size_t)y * S] = s->hbuf[eqf_scene_off(s, x, y)];
+  return EQF_OK;
+}
+
+// Reference initial state (fluid.rs:93-110, 527-570, 602-606), on the host.
+static void host_init(int S, int times, std::vector<std::vector<float>>& F, std::vector<uint8_t>& m) {
+  const size_t n = (size_t)S * S;
+  F.assign(NFIELDS, std::vector<float>(n, 0.0f));
+  m.assign(n, EQF_NO_WALL);
+  auto idx = [S](int64_t x, int64_t y) {
+    x = std::min<int64_t>(std::max<int64_t>(x, 0), S - 1);
+    y = std::min<int64_t>(std::max<int64_t>(y, 0), S - 1);
+    return (size_t)(x + y * S);
+  };
+  const uint32_t su = (uint32_t)S;
+  for (int t = 0; t < times; ++t) {
+    for (uint32_t j = 0; j < su; ++j)  // init_velocities
+      for (uint32_t i = 0; i < su; ++i) {
+        F[F_U][idx(i, j)] += 1.0f;
+        F[F_V][idx(i, j)] += 1.0f;
+      }
+    for (uint32_t j = 0; j < su; ++j)  // init_density, first loop adds 0.0
+      for (uint32_t i = 0; i < su; ++i) {
+        F[F_D][idx(i, j)] += 0.0f;
+        F[F_SC][idx(i, j)] += 0.0f;
+      }
+    // `size/2 - 10 ..= size/2 + 10` in u32: wraps (empty range) below size 20
+    const uint32_t lo = su / 2u - 10u, hi = su / 2u + 10u;
+    for (uint64_t j = lo; j <= hi; ++j)
+      for (uint64_t i = lo; i <= hi; ++i) {
+        F[F_D][idx((int64_t)i, (int64_t)j)] += 0.9f;
+        F[F_SC][idx((int64_t)i, (int64_t)j)] += 0.9f;
+      }
+    for (int i = 0; i < S; ++i) {  // init_walls
+      m[idx(i, 0)] = EQF_DEFAULT_WALL;
+      m[idx(i, S - 1)] = EQF_DEFAULT_WALL;
+    }
+    for (int j = 0; j < S; ++j) {
+      m[idx(0, j)] = EQF_DEFAULT_WALL;
+      m[idx(S - 1, j)] = EQF_DEFAULT_WALL;
+    }
+  }
+}
+
+// Boundary codes and set_boundaries lists derived from the mask.
+static int rebuild_bnd(eqf_scene* s) {
+  const int S = s->S, N = s->N;
+  const auto& m = s->mask;
+  auto wall = [&](int x, int y) { return m[(size_t)x + (size_t)y * S] == EQF_DEFAULT_WALL; };
+  std::vector<uint8_t> colf(S, 0), rowf(S, 0);
+  for (int y = 0; y < S; ++y)
+    for (int x = 0; x < S; ++x)
+      if (!wall(x, y)) {
+        colf[x] = 1;
+        rowf[y] = 1;
+      }
+  const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * N * 64 : (size_t)S * S;
+  std::vector<uint8_t> code(ncode, 0);
+  std::vector<int2> lrow, lcol, lpas;
+  auto off = [&](int x, int y) { return (int)eqf_scene_off(s, x, y); };
+  for (int y = 1; y <= N; ++y)
+    for (int x = 1; x <= N; ++x) {
+      uint8_t c = 0;
+      const bool fl = !wall(x, y);
+      if (fl) c |= C_FLUID;
+      if (wall(x - 1, y)) c |= C_LWALL;
+      if (wall(x + 1, y)) c |= C_RWALL;
+      if (wall(x, y + 1)) c |= C_DWALL;
+      if (wall(x, y - 1)) c |= C_UWALL;
+      if (colf[x]) c |= C_COLF;
+      if (s->layout == L_CYCLIC && ((y - 1) & 63) == 0 && y > 1) {
+        // (x, y-1) is the last row of the strip above
+        if (!wall(x, y - 1) && wall(x, y) && !wall(x, y - 2)) c |= C_FIXUP;
+      }
+      code[s->layout == L_CYCLIC ? (size_t)cyc_off(x, y, S, N, s->ns) : (size_t)x + (size_t)y * S] = c;
+      if (fl) {
+        // AdjustRow: left then right, right wins (fluid.rs:151-164)
+        if (wall(x + 1, y)) lrow.push_back({off(x, y), off(x + 1, y)});
+        else if (wall(x - 1, y)) lrow.push_back({off(x, y), off(x - 1, y)});
+        // AdjustColumn: down (j+1) then up (j-1), up wins (fluid.rs:165-177)
+        if (wall(x, y - 1)) lcol.push_back({off(x, y), off(x, y - 1)});
+        else if (wall(x, y + 1)) lcol.push_back({off(x, y), off(x, y + 1)});
+      }
+    }
+  // Passive: ring copies for every column / row holding a NoWall cell (fluid.rs:179-187)
+  for (int x = 1; x <= N; ++x)
+    if (colf[x]) {
+      lpas.push_back({off(x, 0), off(x, 1)});
+      lpas.push_back({off(x, S - 1), off(x, S - 2)});
+    }
+  for (int y = 1; y <= N; ++y)
+    if (rowf[y]) {
+      lpas.push_back({off(0, y), off(1, y)});
+      lpas.push_back({off(S - 1, y), off(S - 2, y)});
+    }
+  std::vector<int2> all;
+  all.insert(all.end(), lrow.begin(), lrow.end());
+  all.insert(all.end(), lcol.begin(), lcol.end());
+  all.insert(all.end(), lpas.begin(), lpas.end());
+  s->nbnd[0] = (int)lrow.size();
+  s->nbnd[1] = (int)lcol.size();
+  s->nbnd[2] = (int)lpas.size();
+  if (all.empty()) all.push_back({0, 0});
+  if (all.size() > s->bnd_cap) {
+    if (s->d_bnd) HIPCHK(hipFree(s->d_bnd));
+    s->d_bnd = nullptr;
+    HIPCHK(hipMalloc(&s->d_bnd, sizeof(int2) * all.size()));
+    s->bnd_cap = all.size();
+  }
+  HIPCHK(hipMemcpyAsync(s->d_bnd, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->d_code, code.data(), ncode, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->d_rowf, rowf.data(), S, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->bnd_dirty = false;
+  return EQF_OK;
+}
+
+static void fill_desc(eqf_scene* s) {
+  SceneDesc& d = s->desc;
+  const eqf_params& p = s->p;
+  d.fld = s->d_fld;
+  d.src = s->d_src;
+  d.code = s->d_code;
+  d.rowf = s->d_rowf;
+  d.bnd = s->d_bnd;
+  d.hraw = s->d_hraw;
+  d.fstride = s->fstride;
+  d.S = s->S;
+  d.N = s->N;
+  d.ns = s->ns;
+  for (int i = 0; i < 3; ++i) d.nbnd[i] = s->nbnd[i];
+  d.diff_iters = (int)resolved_gs(p);
+  d.press_iters = (int)resolved_press(p);
+  // diffuse (fluid.rs:286-294) and lin_solve's c_recip (fluid.rs:311), in f32
+  const float sf = (float)(s->S - 2);
+  float a = p.delta_t * p.viscosity * sf * sf;
+  d.a_visc = a;
+  d.cr_visc = 1.0f / (1.0f + 4.0f * a);
+  a = p.delta_t * p.diffusion * sf * sf;
+  d.a_diff = a;
+  d.cr_diff = 1.0f / (1.0f + 4.0f * a);
+  d.dt0 = p.delta_t * (float)(s->S - 2);  // fluid.rs:390
+  d.sf = (float)s->S;
+  d.sfm1 = d.sf - 1.0f;
+  d.has_src = s->d_src != nullptr;
+}
+
+static int sync_desc(eqf_scene* s) {
+  if (s->bnd_dirty) {
+    int rc = rebuild_bnd(s);
+    if (rc) return rc;
+  }
+  fill_desc(s);
+  HIPCHK(hipMemcpyAsync(s->d_desc, &s->desc, sizeof(SceneDesc), hipMemcpyHostToDevice, s->stream));
+  return EQF_OK;
+}
+
+static int check_params(const eqf_params* p) {
+  if (!p) return fail(EQF_EINVAL, "params is NULL");
+  if (p->size < 3) return fail(EQF_EINVAL, "size must be >= 3 (one interior cell inside the wall ring)");
+  if (p->size > 32768) return fail(EQF_EINVAL, "size must be <= 32768");
+  if (p->gs_mode != EQF_GS_WAVEFRONT && p->gs_mode != EQF_GS_REDBLACK)
+    return fail(EQF_EINVAL, "gs_mode must be EQF_GS_WAVEFRONT or EQF_GS_REDBLACK");
+  if (p->init_times < 0 || p->init_times > 2) return fail(EQF_EINVAL, "init_times must be 0, 1 or 2");
+  const int64_t it = resolved_gs(*p), pi = resolved_press(*p);
+  if (it > (1 << 24) || pi > (1 << 24)) return fail(EQF_EINVAL, "iteration count too large");
+  if (p->gs_mode == EQF_GS_WAVEFRONT) {
+    const int N = (int)p->size - 2, ns = (N + 63) / 64;
+    if (ns > 16)
+      return fail(EQF_EINVAL, "wavefront mode supports size <= 1026 (16 strips of 64 rows per workgroup); "
+                              "use EQF_GS_REDBLACK for larger grids");
+    if ((int64_t)(ns - 1) * 87 + 64 + std::max(it, pi) * N + 64 > (int64_t)INT32_MAX)
+      return fail(EQF_EINVAL, "iteration count too large for this size");
+  }
+  return EQF_OK;
+}
+
+extern "C" int eqf_ctx_create(int device, eqf_ctx** out) {
+  if (!out) return fail(EQF_EINVAL, "out is NULL");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(EQF_EINVAL, "no such device");
+  HIPCHK(hipSetDevice(device));
+  auto* c = new (std::nothrow) eqf_ctx;
+  if (!c) return fail(EQF_ENOMEM, "host allocation failed");
+  c->device = device;
+  *out = c;
+  return EQF_OK;
+}
+
+extern "C" int eqf_ctx_destroy(eqf_ctx* ctx) {
+  delete ctx;
+  return EQF_OK;
+}
+
+static void scene_free(eqf_scene* s) {
+  if (!s) return;
+  if (s->rb) rb_destroy(s->rb);
+  (void)hipFree(s->d_fld);
+  (void)hipFree(s->d_src);
+  (void)hipFree(s->d_code);
+  (void)hipFree(s->d_rowf);
+  (void)hipFree(s->d_bnd);
+  (void)hipFree(s->d_hraw);
+  (void)hipFree(s->d_desc);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* s = new (std::nothrow) eqf_scene;
+  if (!s) return fail(EQF_ENOMEM, "host allocation failed");
+  s->ctx = ctx;
+  s->p = *p;
+  s->S = (int)p->size;
+  s->N = s->S - 2;
+  s->ns = (s->N + 63) / 64;
+  s->layout = p->gs_mode == EQF_GS_WAVEFRONT ? L_CYCLIC : L_ROWMAJOR;
+  s->fstride = s->layout == L_CYCLIC ? cyc_field_floats(s->S, s->N, s->ns) : rm_field_floats(s->S);
+  hipError_t e = hipStreamCreate(&s->stream);
+  if (e != hipSuccess) {
+    scene_free(s);
+    return fail(EQF_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->N * 64 : (size_t)s->S * s->S;
+  e = hipMalloc(&s->d_fld, sizeof(float) * NFIELDS * s->fstride);
+  if (e == hipSuccess) e = hipMalloc(&s->d_code, ncode);
+  if (e == hipSuccess) e = hipMalloc(&s->d_rowf, s->S);
+  if (e == hipSuccess) e = hipMalloc(&s->d_hraw, sizeof(float) * (size_t)s->ns * s->N + 4);
+  if (e == hipSuccess) e = hipMalloc(&s->d_desc, sizeof(SceneDesc));
+  if (e == hipSuccess) e = hipMemsetAsync(s->d_fld, 0, sizeof(float) * NFIELDS * s->fstride, s->stream);
+  if (e != hipSuccess) {
+    scene_free(s);
+    return fail(e == hipErrorOutOfMemory ? EQF_ENOMEM : EQF_EDEVICE,
+                std::string("device allocation failed: ") + hipGetErrorString(e));
+  }
+  *out = s;
+  return EQF_OK;
+}
+
+extern "C" int eqf_scene_create(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
+  if (!ctx || !out) return fail(EQF_EINVAL, "ctx/out is NULL");
+  eqf_scene* s = nullptr;
+  int rc = scene_alloc(ctx, p, &s);
+  if (rc) return rc;
+  std::vector<std::vector<float>> F;
+  host_init(s->S, p->init_times, F, s->mask);
+  if (p->init_times == 0) s->mask.assign((size_t)s->S * s->S, EQF_NO_WALL);
+  for (int f = 0; f < NFIELDS && !rc; ++f) rc = upload_rm(s, field_ptr(s, f), F[f].data());
+  if (!rc && s->layout == L_ROWMAJOR) {
+    std::string err;
+    s->rb = rb_create(s, err);
+    if (!s->rb) rc = fail(EQF_ENOMEM, err);
+  }
+  if (rc) {
+    scene_free(s);
+    return rc;
+  }
+  s->bnd_dirty = true;
+  *out = s;
+  return EQF_OK;
+}
+
+extern "C" int eqf_scene_clone(const eqf_scene* src, eqf_scene** out) {
+  if (!src || !out) return fail(EQF_EINVAL, "src/out is NULL");
+  eqf_scene* s = nullptr;
+  int rc = scene_alloc(src->ctx, &src->p, &s);
+  if (rc) return rc;
+  s->mask = src->mask;
+  s->bnd_dirty = true;
+  hipError_t e = hipStreamSynchronize(src->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->d_fld, src->d_fld, sizeof(float) * NFIELDS * s->fstride, hipMemcpyDeviceToDevice,
+                       s->stream);
+  if (e == hipSuccess && src->d_src) {
+    e = hipMalloc(&s->d_src, sizeof(float) * 3 * s->fstride);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s->d_src, src->d_src, sizeof(float) * 3 * s->fstride, hipMemcpyDeviceToDevice, s->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    scene_free(s);
+    return fail(EQF_EDEVICE, std::string("clone: ") + hipGetErrorString(e));
+  }
+  if (s->layout == L_ROWMAJOR) {
+    std::string err;
+    s->rb = rb_create(s, err);
+    if (!s->rb) {
+      scene_free(s);
+      return fail(EQF_ENOMEM, err);
+    }
+  }
+  *out = s;
+  return EQF_OK;
+}
+
+extern "C" int eqf_scene_destroy(eqf_scene* s) {
+  if (!s) return EQF_OK;
+  if (s->in_batch) return fail(EQF_EBUSY, "scene is part of a batch");
+  (void)hipStreamSynchronize(s->stream);
+  scene_free(s);
+  return EQF_OK;
+}
+
+extern "C" int eqf_scene_get_params(const eqf_scene* s, eqf_params* out) {
+  if (!s || !out) return fail(EQF_EINVAL, "NULL argument");
+  *out = s->p;
+  return EQF_OK;
+}
+
+extern "C" int eqf_scene_set_params(eqf_scene* s, const eqf_params* p) {
+  if (!s || !p) return fail(EQF_EINVAL, "NULL argument");
+  if (p->size != s->p.size || p->gs_mode != s->p.gs_mode)
+    return fail(EQF_EINVAL, "size and gs_mode are fixed for the lifetime of a scene");
+  int rc = check_params(p);
+  if (rc) return rc;
+  s->p = *p;
+  return EQF_OK;
+}
+
+extern "C" int eqf_rect_validate(int64_t x0, int64_t y0, int64_t x1, int64_t y1, uint32_t size) {
+  const int64_t sz = size;
+  const bool ok = x0 != x1 && y0 != y1 && x0 < x1 && y0 < y1 && x0 < sz && y0 < sz && x1 < sz && y1 < sz;
+  return ok ? EQF_OK : fail(EQF_EINVAL, "Invalid input for Rectangle");
+}
+
+extern "C" int eqf_fill_rect(eqf_scene* s, int64_t x0, int64_t y0, int64_t x1, int64_t y1) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (s->in_batch) return fail(EQF_EBUSY, "scene is part of a batch");
+  const int64_t S = s->S;
+  auto cl = [S](int64_t v) { return std::min<int64_t>(std::max<int64_t>(v, 0), S - 1); };
+  for (int64_t x = x0; x < x1; ++x)
+    for (int64_t y = y0; y < y1; ++y) s->mask[(size_t)(cl(x) + cl(y) * S)] = EQF_DEFAULT_WALL;
+  s->bnd_dirty = true;
+  return EQF_OK;
+}
+
+extern "C" int eqf_set_mask(eqf_scene* s, const uint8_t* mask) {
+  if (!s || !mask) return fail(EQF_EINVAL, "NULL argument");
+  if (s->in_batch) return fail(EQF_EBUSY, "scene is part of a batch");
+  const int S = s->S;
+  for (int i = 0; i < S * S; ++i)
+    if (mask[i] != EQF_DEFAULT_WALL && mask[i] != EQF_NO_WALL)
+      return fail(EQF_EINVAL, "mask bytes must be EQF_DEFAULT_WALL (0) or EQF_NO_WALL (1)");
+  for (int i = 0; i < S; ++i)
+    if (mask[i] != EQF_DEFAULT_WALL || mask[i + (S - 1) * S] != EQF_DEFAULT_WALL || mask[i * S] != EQF_DEFAULT_WALL ||
+        mask[S - 1 + i * S] != EQF_DEFAULT_WALL)
+      return fail(EQF_EINVAL, "the outer ring of the mask must be EQF_DEFAULT_WALL (init_walls)");
+  s->mask.assign(mask, mask + (size_t)S * S);
+  s->bnd_dirty = true;
+  return EQF_OK;
+}
+
+extern "C" int eqf_get_mask(const eqf_scene* s, uint8_t* mask) {
+  if (!s || !mask) return fail(EQF_EINVAL, "NULL argument");
+  std::memcpy(mask, s->mask.data(), s->mask.size());
+  return EQF_OK;
+}
+
+__global__ void eqf_add_cell_kernel(float* a, float* b, float va, float vb) {
+  a[0] += va;
+  if (b) b[0] += vb;
+}
+
+__global__ void eqf_add_array_kernel(float* dst0, float* dst1, const float* src, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) {
+    const float v = src[e];
+    dst0[e] += v;
+    if (dst1) dst1[e] += v;
+  }
+}
+
+static int check_field(int f) {
+  if (f < 0 || f >= NFIELDS) return fail(EQF_EINVAL, "field out of range");
+  return EQF_OK;
+}
+
+static int64_t clamp_cell(const eqf_scene* s, uint32_t x, uint32_t y) {
+  const uint32_t S = (uint32_t)s->S;
+  return eqf_scene_off(s, (int)std::min(x, S - 1), (int)std::min(y, S - 1));
+}
+
+extern "C" int eqf_add_density(eqf_scene* s, uint32_t x, uint32_t y, float amount) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  const int64_t o = clamp_cell(s, x, y);
+  hipLaunchKernelGGL(eqf_add_cell_kernel, dim3(1), dim3(1), 0, s->stream, field_ptr(s, F_D) + o,
+                     field_ptr(s, F_SC) + o, amount, amount);
+  HIPCHK(hipGetLastError());
+  return EQF_OK;
+}
+
+extern "C" int eqf_add_velocity(eqf_scene* s, uint32_t x, uint32_t y, float ax, float ay) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  const int64_t o = clamp_cell(s, x, y);
+  hipLaunchKernelGGL(eqf_add_cell_kernel, dim3(1), dim3(1), 0, s->stream, field_ptr(s, F_U) + o,
+                     field_ptr(s, F_V) + o, ax, ay);
+  HIPCHK(hipGetLastError());
+  return EQF_OK;
+}
+
+// Converts a row-major host array to the scene layout in a device staging buffer.
+static int stage_source(eqf_scene* s, const float* amounts, float** dstage) {
+  const int S = s->S;
+  s->hbuf.assign(s->fstride, 0.0f);
+  for (int y = 0; y < S; ++y)
+    for (int x = 0; x < S; ++x) s->hbuf[eqf_scene_off(s, x, y)] = amounts[(size_t)x + (size_t)y * S];
+  HIPCHK(hipMalloc(dstage, sizeof(float) * s->fstride));
+  HIPCHK(hipMemcpyAsync(*dstage, s->hbuf.data(), sizeof(float) * s->fstride, hipMemcpyHostToDevice, s->stream));
+  return EQF_OK;
+}
+
+extern "C" int eqf_add_source(eqf_scene* s, int field, const float* amounts) {
+  if (!s || !amounts) return fail(EQF_EINVAL, "NULL argument");
+  if (field != EQF_DENSITY && field != EQF_VX && field != EQF_VY)
+    return fail(EQF_EINVAL, "sources exist for EQF_DENSITY, EQF_VX and EQF_VY");
+  float* st = nullptr;
+  int rc = stage_source(s, amounts, &st);
+  if (rc) {
+    (void)hipFree(st);
+    return rc;
+  }
+  float* d0 = field_ptr(s, field);
+  float* d1 = field == EQF_DENSITY ? field_ptr(s, F_SC) : nullptr;
+  const int64_t n = s->fstride;
+  hipLaunchKernelGGL(eqf_add_array_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, d0, d1, st,
+                     n);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  (void)hipFree(st);
+  if (e != hipSuccess) return fail(EQF_EDEVICE, std::string("add_source: ") + hipGetErrorString(e));
+  return EQF_OK;
+}
+
+extern "C" int eqf_set_frame_source(eqf_scene* s, int field, const float* amounts) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (s->in_batch) return fail(EQF_EBUSY, "scene is part of a batch");
+  if (field != EQF_DENSITY && field != EQF_VX && field != EQF_VY)
+    return fail(EQF_EINVAL, "sources exist for EQF_DENSITY, EQF_VX and EQF_VY");
+  const int slot = field == EQF_DENSITY ? 0 : field == EQF_VX ? 1 : 2;
+  if (!s->d_src) {
+    if (!amounts) return EQF_OK;
+    HIPCHK(hipMalloc(&s->d_src, sizeof(float) * 3 * s->fstride));
+    HIPCHK(hipMemsetAsync(s->d_src, 0, sizeof(float) * 3 * s->fstride, s->stream));
+  }
+  float* dst = s->d_src + (int64_t)slot * s->fstride;
+  if (!amounts) {
+    HIPCHK(hipMemsetAsync(dst, 0, sizeof(float) * s->fstride, s->stream));
+  } else {
+    const int S = s->S;
+    s->hbuf.assign(s->fstride, 0.0f);
+    for (int y = 0; y < S; ++y)
+      for (int x = 0; x < S; ++x) s->hbuf[eqf_scene_off(s, x, y)] = amounts[(size_t)x + (size_t)y * S];
+    HIPCHK(hipMemcpyAsync(dst, s->hbuf.data(), sizeof(float) * s->fstride, hipMemcpyHostToDevice, s->stream));
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return EQF_OK;
+}
+
+extern "C" int eqf_upload(eqf_scene* s, int field, const float* host) {
+  if (!s || !host) return fail(EQF_EINVAL, "NULL argument");
+  int rc = check_field(field);
+  if (rc) return rc;
+  return upload_rm(s, field_ptr(s, field), host);
+}
+
+extern "C" int eqf_download(eqf_scene* s, int field, float* host) {
+  if (!s || !host) return fail(EQF_EINVAL, "NULL argument");
+  int rc = check_field(field);
+  if (rc) return rc;
+  return download_rm(s, field_ptr(s, field), host);
+}
+
+static int waves_for(int ns) { return std::max(1, std::min(16, ns)); }
+
+extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (s->in_batch) return fail(EQF_EBUSY, "scene is part of a batch");
+  if (nframes < 0) return fail(EQF_EINVAL, "nframes < 0");
+  if (nframes == 0) return EQF_OK;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  int rc = sync_desc(s);
+  if (rc) return rc;
+  if (s->layout == L_ROWMAJOR) {
+    std::string err;
+    rc = rb_step(s, nframes, err);
+    return rc ? fail(rc, err) : EQF_OK;
+  }
+  const int W = waves_for(s->ns);
+  while (nframes > 0) {
+    const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
+    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 1024>), dim3(1), dim3(64 * W), 0, s->stream, s->d_desc, nf);
+    HIPCHK(hipGetLastError());
+    nframes -= nf;
+  }
+  return EQF_OK;
+}
+
+extern "C" int eqf_sync(eqf_scene* s) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return EQF_OK;
+}
+
+// ------------------------------------------------------------------ batch --
+struct eqf_batch {
+  eqf_ctx* ctx = nullptr;
+  std::vector<eqf_scene*> scenes;
+  SceneDesc* d_descs = nullptr;
+  int W = 1;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_frames = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+};
+
+extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, eqf_batch** out) {
+  if (!ctx || !scenes || n <= 0 || !out) return fail(EQF_EINVAL, "bad arguments");
+  for (int i = 0; i < n; ++i) {
+    if (!scenes[i]) return fail(EQF_EINVAL, "NULL scene");
+    if (scenes[i]->layout != L_CYCLIC) return fail(EQF_EINVAL, "batches hold wavefront-mode scenes");
+    if (scenes[i]->in_batch) return fail(EQF_EBUSY, "scene already in a batch");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  auto* b = new eqf_batch;
+  b->ctx = ctx;
+  // Largest scenes first: their workgroups are the longest, so they are
+  // dispatched first (longest-processing-time order) and spread over XCDs.
+  b->scenes.assign(scenes, scenes + n);
+  std::stable_sort(b->scenes.begin(), b->scenes.end(), [](const eqf_scene* a, const eqf_scene* c) {
+    return (int64_t)a->N * a->N * std::max(a->desc.diff_iters, 1) > (int64_t)c->N * c->N * std::max(c->desc.diff_iters, 1);
+  });
+  std::vector<SceneDesc> descs(n);
+  int maxns = 1;
+  for (int i = 0; i < n; ++i) {
+    eqf_scene* s = b->scenes[i];
+    int rc = sync_desc(s);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+    descs[i] = s->desc;
+    maxns = std::max(maxns, s->ns);
+  }
+  b->W = waves_for(maxns);
+  hipError_t e = hipStreamCreate(&b->stream);
+  if (e == hipSuccess) e = hipMalloc(&b->d_descs, sizeof(SceneDesc) * n);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_descs, descs.data(), sizeof(SceneDesc) * n, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(b->d_descs);
+    delete b;
+    return fail(EQF_EDEVICE, std::string("batch create: ") + hipGetErrorString(e));
+  }
+  for (auto* s : b->scenes) s->in_batch = 1;
+  *out = b;
+  return EQF_OK;
+}
+
+static hipError_t batch_launch(eqf_batch* b, int nframes_per_launch) {
+  const int n = (int)b->scenes.size();
+  if (b->W <= 8)
+    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 512>), dim3(n), dim3(64 * b->W), 0, b->stream, b->d_descs,
+                       nframes_per_launch);
+  else
+    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 1024>), dim3(n), dim3(64 * b->W), 0, b->stream, b->d_descs,
+                       nframes_per_launch);
+  return hipGetLastError();
+}
+
+extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {
+  if (!b) return fail(EQF_EINVAL, "batch is NULL");
+  if (nframes < 0) return fail(EQF_EINVAL, "nframes < 0");
+  if (nframes == 0) return EQF_OK;
+  HIPCHK(hipSetDevice(b->ctx->device));
+  // One launch per frame, captured once as a graph of `nframes` launches.
+  if (!b->gexec || b->graph_frames != nframes) {
+    if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t le = hipSuccess;
+    for (int64_t f = 0; f < nframes && le == hipSuccess; ++f) le = batch_launch(b, 1);
+    hipError_t ce = hipStreamEndCapture(b->stream, &g);
+    if (le != hipSuccess) return fail(EQF_EDEVICE, std::string("batch capture: ") + hipGetErrorString(le));
+    HIPCHK(ce);
+    HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    b->graph_frames = nframes;
+  }
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  return EQF_OK;
+}
+
+extern "C" int eqf_batch_sync(eqf_batch* b) {
+  if (!b) return fail(EQF_EINVAL, "batch is NULL");
+  HIPCHK(hipStreamSynchronize(b->stream));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->last_ms = ms;
+  return EQF_OK;
+}
+
+extern "C" int eqf_batch_last_ms(const eqf_batch* b, double* ms) {
+  if (!b || !ms) return fail(EQF_EINVAL, "NULL argument");
+  *ms = b->last_ms;
+  return EQF_OK;
+}
+
+extern "C" int eqf_batch_destroy(eqf_batch* b) {
+  if (!b) return EQF_OK;
+  (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  for (auto* s : b->scenes) s->in_batch = 0;
+  (void)hipFree(b->d_descs);
+  (void)hipEventDestroy(b->ev0);
+  (void)hipEventDestroy(b->ev1);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+  return EQF_OK;
+}

@@ -1,0 +1,142 @@
+/*
+ * eqfluid.h -- C ABI of libeqfluid.so, the MI355X (gfx950) Stam frame step.
+ *
+ * Drop-in boundary for the reference's simulation core, `Fluid`
+ * (src/simulation/fluid.rs).  The reference has no FFI layer: its boundary is
+ * the Rust API of `Fluid` that `CurrentSimulation::simulate`
+ * (src/simulation/renderer_helpers.rs:52-72) drives once per frame.  Each
+ * entry point below names the reference item it replaces.  Fields stay
+ * resident on the GPU; host buffers are only borrowed for the duration of a
+ * call.  Every function returns an eqf_status; on failure eqf_last_error()
+ * gives a message (thread-local).  No function aborts or unwinds.
+ *
+ * Layout of every host array: row-major `idx(x, y) = x + y*size`
+ * (fluid.rs:31-35), size*size elements, ring included.
+ */
+#ifndef EQFLUID_H
+#define EQFLUID_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EQF_ABI_VERSION 1
+
+typedef enum eqf_status {
+    EQF_OK = 0,
+    EQF_EINVAL = 1,   /* bad argument (reference: panics, e.g. obstacle.rs:127) */
+    EQF_ENOMEM = 2,   /* device or host allocation failed / budget exceeded */
+    EQF_EDEVICE = 3,  /* HIP runtime error */
+    EQF_EBUSY = 4,    /* object in use (scene owned by a batch or server) */
+    EQF_ENOTFOUND = 5 /* unknown ticket */
+} eqf_status;
+
+/* The six fields of `Fluid` (fluid.rs:59-77). */
+typedef enum eqf_field {
+    EQF_DENSITY = 0,    /* pub density */
+    EQF_SCRATCH = 1,    /* scratch_space (private in the reference) */
+    EQF_VX = 2,         /* pub velocities_x */
+    EQF_VY = 3,         /* pub velocities_y */
+    EQF_VX0 = 4,        /* velocities_x0 (private) */
+    EQF_VY0 = 5         /* velocities_y0 (private) */
+} eqf_field;
+
+/* `ContainerWall` (fluid.rs:12-17), one byte per cell, declaration order. */
+enum { EQF_DEFAULT_WALL = 0, EQF_NO_WALL = 1 };
+
+/* Gauss-Seidel ordering of every lin_solve (fluid.rs:301-325). */
+typedef enum eqf_gs_mode {
+    EQF_GS_WAVEFRONT = 0, /* lexicographic order of the reference: bit-exact */
+    EQF_GS_REDBLACK = 1   /* red-black order: parallel, checked against FP64 */
+} eqf_gs_mode;
+
+/* `SimulationConfigs` (configs.rs:5-12) + `FluidConfigs` (configs.rs:37-48)
+ * minus the colours, plus the GPU knobs. */
+typedef struct eqf_params {
+    uint32_t size;          /* SimulationConfigs.size, ring included, >= 3 */
+    float delta_t;          /* SimulationConfigs.delta_t */
+    int64_t frames;         /* SimulationConfigs.frames */
+    float diffusion;        /* FluidConfigs.diffusion */
+    float viscosity;        /* FluidConfigs.viscousity */
+    int64_t gs_iters;       /* sweeps per diffuse lin_solve; < 0: `frames`, as the reference */
+    int64_t pressure_iters; /* sweeps per project lin_solve; < 0: same as diffuse */
+    int32_t gs_mode;        /* eqf_gs_mode */
+    int32_t init_times;     /* 1: state of Fluid::new (fluid.rs:93); 2: Fluid::default (fluid.rs:83) */
+} eqf_params;
+
+typedef struct eqf_ctx eqf_ctx;
+typedef struct eqf_scene eqf_scene;
+typedef struct eqf_batch eqf_batch;
+typedef struct eqf_server eqf_server;
+
+/* Defaults of SimulationConfigs::default (configs.rs:15-21) and
+ * FluidConfigs::default (configs.rs:51-59); init_times = 1. */
+void eqf_params_default(eqf_params *p);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *eqf_last_error(void);
+
+/* One context per GPU: owns the device selection. */
+int eqf_ctx_create(int device, eqf_ctx **out);
+int eqf_ctx_destroy(eqf_ctx *ctx);
+
+/* Fluid::new / Fluid::default (fluid.rs:83-110): allocates the six fields and
+ * the mask on the device and writes the reference's initial state. */
+int eqf_scene_create(eqf_ctx *ctx, const eqf_params *p, eqf_scene **out);
+/* #[derive(Clone)] on Fluid (fluid.rs:51): device-to-device deep copy. */
+int eqf_scene_clone(const eqf_scene *src, eqf_scene **out);
+int eqf_scene_destroy(eqf_scene *s);
+int eqf_scene_get_params(const eqf_scene *s, eqf_params *out);
+/* Changes delta_t, diffusion, viscosity, iteration counts; size and mode are fixed. */
+int eqf_scene_set_params(eqf_scene *s, const eqf_params *p);
+
+/* Rectangle::are_all_points_valid (obstacle.rs:134-147); EQF_EINVAL with
+ * "Invalid input for Rectangle" (obstacle.rs:127) when invalid. */
+int eqf_rect_validate(int64_t x0, int64_t y0, int64_t x1, int64_t y1, uint32_t size);
+/* Fluid::fill_obstacle (fluid.rs:610-619): walls on [x0,x1) x [y0,y1), clamped. */
+int eqf_fill_rect(eqf_scene *s, int64_t x0, int64_t y0, int64_t x1, int64_t y1);
+/* pub cells_type (fluid.rs:80): read or replace the whole mask.  The ring
+ * must stay DefaultWall (init_walls, fluid.rs:552-570). */
+int eqf_set_mask(eqf_scene *s, const uint8_t *mask);
+int eqf_get_mask(const eqf_scene *s, uint8_t *mask);
+
+/* add_density / add_velocity (fluid.rs:120-131), made public: one cell,
+ * clamped index; density is added to the density and scratch fields. */
+int eqf_add_density(eqf_scene *s, uint32_t x, uint32_t y, float amount);
+int eqf_add_velocity(eqf_scene *s, uint32_t x, uint32_t y, float ax, float ay);
+/* add_density(x, y, a[idx]) for every cell (field EQF_DENSITY), or
+ * add_velocity with one component (EQF_VX / EQF_VY). */
+int eqf_add_source(eqf_scene *s, int field, const float *amounts);
+/* Same as eqf_add_source, but applied on the device before every frame of
+ * eqf_step / batch / server (the frame loop of renderer_helpers.rs:54-59
+ * calls add_noise before step).  NULL clears it. */
+int eqf_set_frame_source(eqf_scene *s, int field, const float *amounts);
+
+/* Read / write a whole field (the pub Vec<f32> fields, fluid.rs:61-77). */
+int eqf_upload(eqf_scene *s, int field, const float *host);
+int eqf_download(eqf_scene *s, int field, float *host);
+
+/* Fluid::step (fluid.rs:437-524), `nframes` times, each preceded by the
+ * frame sources.  Asynchronous on the scene's stream. */
+int eqf_step(eqf_scene *s, int64_t nframes);
+/* Waits for the scene's queued work. */
+int eqf_sync(eqf_scene *s);
+
+/* Batches: many scenes (any sizes, wavefront mode) advanced by one fused
+ * launch per frame; K frames are captured once as a HIP graph and replayed.
+ * Scenes stay owned by the caller but must not be stepped separately while
+ * in a batch. */
+int eqf_batch_create(eqf_ctx *ctx, eqf_scene *const *scenes, int n, eqf_batch **out);
+int eqf_batch_step(eqf_batch *b, int64_t nframes);
+int eqf_batch_sync(eqf_batch *b);
+int eqf_batch_destroy(eqf_batch *b);
+/* Wall time of the last eqf_batch_step in milliseconds (HIP events). */
+int eqf_batch_last_ms(const eqf_batch *b, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
