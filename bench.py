@@ -1,0 +1,234 @@
+"""Benchmark of the MI355X Stam frame step.
+
+Headline workload (``value``): the batched mix -- 512 independent scenes, N
+drawn from {64,128,256,512}, viscosity and diffusion log-uniform in
+[1e-6,1e-3], 0-4 rectangle obstacles, 20 Gauss-Seidel sweeps per lin_solve,
+dt 0.1, exact (wavefront, bit-for-bit) mode -- advanced by one fused launch
+per frame, K frames captured as one HIP graph.  A step is one frame of the
+whole batch.  value = sum(N^2) * K / elapsed (interior cells; the ring is not
+counted).
+
+Also reported beside it: the single-grid drop-in on the same mix (one scene
+at a time), the N=1024 single grid, the HBM roofline of the frame kernel
+and the CPU oracle timed on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scenes 512]
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+import numpy as np  # noqa: E402
+
+from equilibrium_amd import _abi, scenes  # noqa: E402
+
+METRIC = "cell-updates/s (N=4096 single + batched mix); p50/p99 frame latency; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def b_alg(iters):
+    """Algorithmic HBM bytes per cell-update (one interior cell, one frame):
+    5 lin_solves x iters sweeps x 12 B (read x, read x0, write x) + 108 B for
+    divergence/p-init (2x16), gradient (2x20), advect u,v (16) and advect
+    density with the scratch copy (20)."""
+    return 60 * iters + 108
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def make_scenes(scs):
+    hs = []
+    for sc in scs:
+        h = _abi.Scene(scenes.scene_params(sc))
+        scenes.apply_to_scene(h, sc)
+        hs.append(h)
+    for h in hs:
+        h.sync()
+    return hs
+
+
+def bench_batch(scs, steps, warmup):
+    hs = make_scenes(scs)
+    b = _abi.Batch(hs)
+    if warmup:
+        b.step(warmup)
+        b.sync()
+    device_sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    device_sync()
+    wall = time.perf_counter() - t0
+    ev_ms = b.last_ms()
+    b.close()
+    for h in hs:
+        h.close()
+    return wall, ev_ms
+
+
+def bench_single_dropin(scs, frames):
+    """The plain drop-in: scenes advanced one at a time with eqf_step."""
+    hs = make_scenes(scs)
+    for h in hs:  # warm-up: one frame each
+        h.step(1)
+        h.sync()
+    device_sync()
+    t0 = time.perf_counter()
+    for h in hs:
+        h.step(frames)
+        h.sync()
+    wall = time.perf_counter() - t0
+    for h in hs:
+        h.close()
+    return wall
+
+
+def cpu_baseline(scs, frames, threads):
+    """The CPU oracle (C restatement of the reference step, -O2, no FMA) on a
+    sample of the same scenes, one scene per thread."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle
+    from helpers import oracle_state
+
+    states = [oracle_state(sc) for sc in scs]
+
+    def run(i):
+        sc = scs[i]
+        states[i].run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], frames,
+                      src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(run, range(len(scs))))
+    wall = time.perf_counter() - t0
+    cells = sum((sc["size"] - 2) ** 2 for sc in scs) * frames
+    return cells / wall, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--no-extras", action="store_true", help="headline batch only (for profiling)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    mix = scenes.config3(n=args.scenes, frames=args.steps)
+    iters = mix[0]["iters"]
+    cells = sum((sc["size"] - 2) ** 2 for sc in mix)
+    log(f"batched mix: {len(mix)} scenes, {cells} interior cells, {iters} sweeps")
+    wall, ev_ms = bench_batch(mix, args.steps, args.warmup)
+    value = cells * args.steps / wall
+    ms_per_step = wall * 1e3 / args.steps
+    launch_ms = ev_ms / args.steps  # HIP events on the batch stream around the K launches
+    alg_bytes = cells * b_alg(iters)
+    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+    log(f"batch: {value:.4g} cell-updates/s, {ms_per_step:.3f} ms/frame, kernel {launch_ms:.3f} ms")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes generated on the host)",
+        "config": {
+            "workload": f"batched mix: {len(mix)} scenes, N uniform in {{64,128,256,512}}, "
+                        f"visc/diff log-U[1e-6,1e-3], 0-4 rectangles, {iters} GS sweeps, dt 0.1, "
+                        "exact wavefront mode, one fused launch per frame (HIP graph)",
+            "scenes": len(mix),
+            "interior_cells": cells,
+            "gs_sweeps": iters,
+            "mode": "wavefront (bit-exact vs CPU oracle)",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "eqf_frames_cyclic (fused frame step, one workgroup per scene)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "alg_bytes_per_launch": alg_bytes,
+            "launch_ms": launch_ms,
+        },
+    }
+
+    if not args.no_extras:
+        # single-grid drop-in on a stratified sample of the same mix
+        sample = mix[::8]
+        sframes = 2
+        swall = bench_single_dropin(sample, sframes)
+        scells = sum((sc["size"] - 2) ** 2 for sc in sample) * sframes
+        single = scells / swall
+        out["single_grid_dropin"] = {"value": single, "unit": "cell-updates/s",
+                                     "sample": f"{len(sample)} scenes of the mix x {sframes} frames, "
+                                               "one eqf_step per scene"}
+        out["batched_over_single"] = value / single
+        log(f"single drop-in: {single:.4g} cell-updates/s; batched/single = {value / single:.1f}x")
+
+        # N=1024 single grid, exact mode
+        c1 = scenes.config1()
+        h = make_scenes([c1])[0]
+        h.step(1)
+        h.sync()
+        nf = 5
+        t0 = time.perf_counter()
+        h.step(nf)
+        h.sync()
+        w1 = time.perf_counter() - t0
+        out["n1024_single_wavefront"] = {"value": 1024 * 1024 * nf / w1, "unit": "cell-updates/s",
+                                         "ms_per_frame": w1 * 1e3 / nf}
+        h.close()
+        log(f"N=1024 single: {1024 * 1024 * nf / w1:.4g} cell-updates/s")
+
+        # CPU baseline: oracle on this host, bounded sample
+        threads = max(1, min(16, os.cpu_count() or 1))
+        probe = mix[:threads]
+        rate_probe, wprobe = cpu_baseline(probe, 1, threads)
+        # size the sample for about cpu_seconds of work
+        per_scene_s = wprobe / max(1, len(probe)) * threads
+        n_cpu = int(max(threads, min(len(mix), args.cpu_seconds / max(per_scene_s, 1e-3) * threads)))
+        cpu_sample = mix[: n_cpu]
+        rate, wcpu = cpu_baseline(cpu_sample, 1, threads)
+        out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"first {len(cpu_sample)} scenes of the mix x 1 frame, "
+                                         f"one scene per thread, {wcpu:.1f} s"}
+        log(f"cpu oracle: {rate:.4g} cell-updates/s on {threads} threads ({wcpu:.1f} s)")
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
