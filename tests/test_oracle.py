@@ -1,0 +1,460 @@
+"""Pins the CPU oracle (oracle/eqf_oracle.c) to the reference.
+
+The reference is a Rust crate that cannot be built here, so its own tests can
+only be re-run through their known answers.  Everything else is pinned by
+cases whose expected values were derived by hand from the reference source
+(cited per test), by closed forms, and by an independent dense linear-algebra
+formulation of the Gauss-Seidel fixed point -- each chosen so that a plausible
+transcription mistake (a sign, an index, an evaluation order, left vs right
+wins, lexicographic vs other order) changes the answer.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import COL, FLUID, LEX, PASSIVE, RB, ROW, WALL
+
+
+def ring_mask(S):
+    m = np.full(S * S, FLUID, np.uint8)
+    for i in range(S):
+        m[i] = m[i + (S - 1) * S] = m[i * S] = m[S - 1 + i * S] = WALL
+    return m
+
+
+# ---------------------------------------------------------------- layout --
+
+def test_idx_round_trip_reference_known_answer():
+    """fluid.rs:626-635: idx!(3,4,10) round-trips through to_coordinate."""
+    i = oracle.idx(3, 4, 10)
+    assert i == 43
+    assert oracle.to_coordinate(i, 10) == (3, 4)
+
+
+def test_idx_clamps_coordinates():
+    """idx! clamps both coordinates into [0, size-1] (fluid.rs:31-35)."""
+    assert oracle.idx(-3, 15, 10) == 0 + 9 * 10
+    assert oracle.idx(12, -1, 10) == 9
+
+
+# ------------------------------------------------------------ obstacles --
+
+def test_default_mark_obstacles_wall_count_1408():
+    """renderer_helpers.rs:222-252: Fluid::default + the default rectangle
+    (80,80)-(110,110) at size 128 gives 2*(128+126) + 30*30 = 1408 walls."""
+    st = oracle.State.new(128, times=2)
+    st.fill_rect(80, 80, 110, 110)
+    assert int(np.sum(st.mask == WALL)) == 1408
+
+
+def test_rectangle_validation_reference_panics():
+    """obstacle.rs:100-115 (both should_panic tests) and the default rectangle."""
+    assert not oracle.rect_valid(50, 120, 127, 110, 128)
+    assert not oracle.rect_valid(12, 12, 10, 10, 128)
+    assert oracle.rect_valid(80, 80, 110, 110, 128)
+    assert not oracle.rect_valid(0, 0, 128, 5, 128)   # coordinate == size
+    assert oracle.rect_valid(0, 0, 127, 5, 128)       # 0 is allowed (obstacle.rs:146)
+    assert not oracle.rect_valid(3, 3, 3, 9, 128)     # degenerate
+
+
+def test_fill_rect_half_open_and_clamped():
+    """fill_obstacle marks [x0,x1) x [y0,y1) with clamped indices (fluid.rs:610-619)."""
+    st = oracle.State.new(10)
+    st.fill_rect(2, 3, 5, 4)
+    m = st.mask.reshape(10, 10)
+    assert m[3, 2:5].tolist() == [WALL] * 3 and m[3, 5] == FLUID and m[4, 2] == FLUID
+    st2 = oracle.State.new(10)
+    st2.fill_rect(-5, 4, 1, 6)   # clamps onto column 0 (already ring)
+    assert np.array_equal(st2.mask, oracle.State.new(10).mask)
+
+
+# ------------------------------------------------------------------ init --
+
+def test_init_new_and_default_known_values():
+    """Fluid::new: U=V=1 everywhere, D=S=0.9 on [S/2-10, S/2+10]^2
+    (fluid.rs:527-548); Fluid::default runs init twice (fluid.rs:83-89)."""
+    S = 128
+    for times, vel, dens in ((1, 1.0, np.float32(0.9)), (2, 2.0, np.float32(0.9) + np.float32(0.9))):
+        st = oracle.State.new(S, times)
+        assert np.all(st.f["vx"] == vel) and np.all(st.f["vy"] == vel)
+        assert np.all(st.f["vx0"] == 0) and np.all(st.f["vy0"] == 0)
+        d = st.f["density"].reshape(S, S)
+        assert np.all(d[54:75, 54:75] == dens)
+        assert int(np.count_nonzero(d)) == 21 * 21
+        assert np.array_equal(st.f["density"], st.f["scratch"])
+        assert int(np.sum(st.mask == WALL)) == 2 * (S + S - 2)
+
+
+def test_init_density_small_sizes_follow_u32_and_clamp():
+    """size/2-10 is u32: it wraps below 20 (empty block, release build); at 20
+    and 21 idx! clamps the block's far edge onto the last column/row, which
+    then receives the 0.9 more than once (fluid.rs:534-537, 31-35)."""
+    assert np.count_nonzero(oracle.State.new(12).f["density"]) == 0
+    S = 20
+    d = oracle.State.new(S).f["density"].reshape(S, S)
+    one = np.float32(0.9)
+    two = np.float32(one + one)
+    four = np.float32(np.float32(two + one) + one)
+    assert d[5, 5] == one and d[5, 19] == two and d[19, 5] == two and d[19, 19] == four
+    assert d[0, 0] == one  # lo = 0 is inside the block
+
+
+# --------------------------------------------------------- set_boundaries --
+
+def _bnd_case(mode):
+    S = 5
+    x = np.arange(1, 26, dtype=np.float64)
+    m = ring_mask(S)
+    m[2 + 2 * S] = WALL
+    oracle.set_bnd(mode, x, S, m)
+    return x.reshape(S, S)   # [y, x]
+
+
+def test_set_bnd_adjust_row_hand_case():
+    """fluid.rs:151-164: left then right, right wins; corners fluid.rs:265-271.
+    x(i,j) = i + 5j + 1 on a 5x5 grid with a wall at (2,2)."""
+    g = _bnd_case(ROW)
+    exp = {(1, 1): -6, (2, 1): 8, (3, 1): -10, (1, 2): -13, (3, 2): -15,
+           (1, 3): -16, (2, 3): 18, (3, 3): -20, (2, 2): 13,
+           (0, 0): 4, (0, 4): 19, (4, 0): 7, (4, 4): 22, (2, 0): 3, (0, 2): 11}
+    for (i, j), v in exp.items():
+        assert g[j, i] == v, (i, j)
+
+
+def test_set_bnd_adjust_column_hand_case():
+    """fluid.rs:165-177: down (j+1) then up (j-1), up wins."""
+    g = _bnd_case(COL)
+    exp = {(1, 1): -2, (2, 1): -3, (3, 1): -4, (1, 2): 12, (3, 2): 14,
+           (1, 3): -22, (2, 3): -13, (3, 3): -24, (0, 0): 4, (4, 4): 22}
+    for (i, j), v in exp.items():
+        assert g[j, i] == v, (i, j)
+
+
+def test_set_bnd_passive_hand_case():
+    """fluid.rs:179-187: ring copies of the neighbouring interior cells for
+    every row/column holding a NoWall cell; interior untouched."""
+    g = _bnd_case(PASSIVE)
+    exp = {(1, 0): 7, (2, 0): 8, (3, 0): 9, (1, 4): 17, (2, 4): 18, (3, 4): 19,
+           (0, 1): 7, (0, 2): 12, (0, 3): 17, (4, 1): 9, (4, 2): 14, (4, 3): 19,
+           (0, 0): 7, (0, 4): 17, (4, 0): 9, (4, 4): 19, (2, 2): 13, (1, 2): 12}
+    for (i, j), v in exp.items():
+        assert g[j, i] == v, (i, j)
+
+
+def test_set_bnd_passive_skips_column_without_fluid():
+    S = 5
+    x = np.arange(1, 26, dtype=np.float64)
+    m = ring_mask(S)
+    for j in range(S):
+        m[2 + j * S] = WALL   # column 2 is all wall
+    oracle.set_bnd(PASSIVE, x, S, m)
+    g = x.reshape(S, S)
+    assert g[0, 2] == 3 and g[4, 2] == 23   # untouched
+    assert g[0, 1] == 7                       # column 1 copied
+
+
+# -------------------------------------------------------------- lin_solve --
+
+def test_lin_solve_a_zero_is_copy_plus_bnd():
+    """diffusion 0 gives a = 0, c = 1: one sweep sets x = x0 (fluid.rs:315-320)."""
+    S = 9
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(S * S).astype(np.float32)
+    x0 = rng.standard_normal(S * S).astype(np.float32)
+    m = ring_mask(S)
+    oracle.lin_solve(PASSIVE, x, x0, np.float32(0.0), np.float32(1.0), S, 3, m)
+    want = x0.copy()
+    oracle.set_bnd(PASSIVE, want, S, m)
+    g, w = x.reshape(S, S), want.reshape(S, S)
+    assert np.array_equal(g[1:-1, 1:-1], w[1:-1, 1:-1])
+    assert np.array_equal(x, want)
+
+
+def test_one_sweep_hand_case_lexicographic_and_red_black():
+    """4x4 grid, a=1, c=4, x0(1,1)=4: the lexicographic order gives
+    (1,1)=1, (2,1)=(1,2)=0.25, (2,2)=0.125; red-black gives (2,2)=0."""
+    S = 4
+    m = ring_mask(S)
+    x0 = np.zeros(S * S, np.float32)
+    x0[1 + S] = 4.0
+    for order, exp22 in ((LEX, 0.125), (RB, 0.0)):
+        x = np.zeros(S * S, np.float32)
+        oracle.lin_solve(PASSIVE, x, x0, np.float32(1.0), np.float32(4.0), S, 1, m, order)
+        g = x.reshape(S, S)
+        assert g[1, 1] == 1.0 and g[1, 2] == 0.25 and g[2, 1] == 0.25 and g[2, 2] == exp22
+        # ring copies and corners
+        assert g[0, 1] == 1.0 and g[1, 0] == 1.0 and g[0, 0] == 1.0
+        assert g[3, 2] == exp22 and g[2, 3] == exp22 and g[3, 3] == exp22
+        assert g[0, 3] == 0.25 and g[3, 0] == 0.25
+
+
+def _gs_fixed_point(mode, x_init, x0, a, c, S, mask):
+    """Dense solve of the converged lexicographic Gauss-Seidel sweep with
+    set_boundaries, written independently of the sweep loop: unknowns are
+    the raw interior values y of a converged sweep; left/up neighbours enter
+    with their raw values, right/down neighbours with their post-boundary
+    values, ring cells with their (constant or copied) values."""
+    cr = 1.0 / c
+    wall = lambda i, j: mask[i + j * S] == WALL
+    interior = [(i, j) for j in range(1, S - 1) for i in range(1, S - 1)]
+    pos = {cell: k for k, cell in enumerate(interior)}
+    n = len(interior)
+    colf = [any(not wall(i, j) for j in range(S)) for i in range(S)]
+    rowf = [any(not wall(i, j) for i in range(S)) for j in range(S)]
+
+    def ring_val(i, j):
+        """Affine form (coef, const) of ring cell (i,j) between sweeps."""
+        if mode == PASSIVE:
+            if j == 0 and 0 < i < S - 1 and colf[i]:
+                return raw(i, 1)
+            if j == S - 1 and 0 < i < S - 1 and colf[i]:
+                return raw(i, S - 2)
+            if i == 0 and 0 < j < S - 1 and rowf[j]:
+                return raw(1, j)
+            if i == S - 1 and 0 < j < S - 1 and rowf[j]:
+                return raw(S - 2, j)
+        return (np.zeros(n), float(x_init[i + j * S]))
+
+    def raw(i, j):
+        v = np.zeros(n)
+        v[pos[(i, j)]] = 1.0
+        return (v, 0.0)
+
+    def is_ring(i, j):
+        return i in (0, S - 1) or j in (0, S - 1)
+
+    def val(i, j):  # value of a cell "between sweeps" (post set_boundaries)
+        return ring_val(i, j) if is_ring(i, j) else post(i, j)
+
+    def neg(f):
+        return (-f[0], -f[1])
+
+    def post(i, j):
+        if wall(i, j) or mode == PASSIVE:
+            return raw(i, j)
+        if mode == ROW:
+            if wall(i + 1, j):
+                return neg(val(i + 1, j) if is_ring(i + 1, j) else raw(i + 1, j))
+            if wall(i - 1, j):
+                return neg(val(i - 1, j) if is_ring(i - 1, j) else raw(i - 1, j))
+        else:
+            if wall(i, j - 1):
+                return neg(val(i, j - 1) if is_ring(i, j - 1) else raw(i, j - 1))
+            if wall(i, j + 1):
+                return neg(val(i, j + 1) if is_ring(i, j + 1) else raw(i, j + 1))
+        return raw(i, j)
+
+    M = np.eye(n)
+    b = np.zeros(n)
+    for k, (i, j) in enumerate(interior):
+        terms = []
+        for (ni, nj, newer) in ((i + 1, j, False), (i - 1, j, True), (i, j + 1, False), (i, j - 1, True)):
+            if is_ring(ni, nj):
+                terms.append(ring_val(ni, nj))
+            elif newer:
+                terms.append(raw(ni, nj))
+            else:
+                terms.append(post(ni, nj))
+        for coef, const in terms:
+            M[k] -= a * cr * coef
+            b[k] += a * cr * const
+        b[k] += cr * x0[i + j * S]
+    y = np.linalg.solve(M, b)
+    out = np.array(x_init, np.float64)
+    for (i, j) in interior:
+        cf, cst = post(i, j)
+        out[i + j * S] = cf @ y + cst
+    for i in range(S):
+        for j in range(S):
+            if is_ring(i, j) and not (i in (0, S - 1) and j in (0, S - 1)):
+                cf, cst = ring_val(i, j)
+                out[i + j * S] = cf @ y + cst
+    out[0] = 0.5 * (out[1] + out[S])
+    out[(S - 1) * S] = 0.5 * (out[1 + (S - 1) * S] + out[(S - 2) * S])
+    out[S - 1] = 0.5 * (out[S - 2] + out[S - 1 + S])
+    out[S * S - 1] = 0.5 * (out[S - 2 + (S - 1) * S] + out[S - 1 + (S - 2) * S])
+    return out
+
+
+@pytest.mark.parametrize("mode", [ROW, COL, PASSIVE])
+def test_lin_solve_converges_to_dense_fixed_point(mode):
+    S = 8
+    rng = np.random.default_rng(7 + mode)
+    m = ring_mask(S)
+    for (i, j) in ((3, 3), (4, 3), (3, 4), (5, 6), (1, 5)):
+        m[i + j * S] = WALL
+    x = rng.standard_normal(S * S)
+    x0 = rng.standard_normal(S * S)
+    a, c = 0.7, 1.0 + 4 * 0.7
+    want = _gs_fixed_point(mode, x, x0, a, c, S, m)
+    got = x.copy()
+    oracle.lin_solve(mode, got, x0, a, c, S, 400, m)
+    assert np.max(np.abs(got - want)) < 1e-10
+
+
+def test_red_black_converges_to_same_passive_fixed_point():
+    """Without obstacles the Passive fixed point does not depend on order."""
+    S = 8
+    rng = np.random.default_rng(3)
+    m = ring_mask(S)
+    x = rng.standard_normal(S * S)
+    x0 = rng.standard_normal(S * S)
+    a, c = 0.9, 1.0 + 4 * 0.9
+    want = _gs_fixed_point(PASSIVE, x, x0, a, c, S, m)
+    got = x.copy()
+    oracle.lin_solve(PASSIVE, got, x0, a, c, S, 400, m, RB)
+    assert np.max(np.abs(got - want)) < 1e-10
+
+
+# ----------------------------------------------------------------- advect --
+
+def test_advect_zero_velocity_copies():
+    """u = v = 0: back-trace lands on the cell, s1 = t1 = 0 (fluid.rs:400-428)."""
+    S = 10
+    rng = np.random.default_rng(2)
+    d0 = rng.standard_normal(S * S).astype(np.float32)
+    d = np.zeros(S * S, np.float32)
+    z = np.zeros(S * S, np.float32)
+    m = ring_mask(S)
+    oracle.advect(PASSIVE, d, d0, z, z, S, np.float32(0.1), m)
+    g, w = d.reshape(S, S), d0.reshape(S, S)
+    assert np.array_equal(g[1:-1, 1:-1], w[1:-1, 1:-1])
+
+
+def test_advect_integer_shift_and_clamped_first_column():
+    """dt0 = dt*(S-2) = 1 and u = 1: x = i-1, so d(i,j) = d0(i-1,j) for i >= 2;
+    at i = 1, x = 0 clamps to 0.5 and averages the ring and column 1
+    (Passive set_boundaries leaves interior cells alone)."""
+    S = 10
+    rng = np.random.default_rng(4)
+    d0 = rng.integers(-8, 8, S * S).astype(np.float32)
+    u = np.ones(S * S, np.float32)
+    v = np.zeros(S * S, np.float32)
+    d = np.zeros(S * S, np.float32)
+    oracle.advect(PASSIVE, d, d0, u, v, S, np.float32(0.125), ring_mask(S))
+    g, w = d.reshape(S, S), d0.reshape(S, S)
+    assert np.array_equal(g[1:-1, 2:-1], w[1:-1, 1:-2])
+    assert np.array_equal(g[1:-1, 1], np.float32(0.5) * w[1:-1, 0] + np.float32(0.5) * w[1:-1, 1])
+
+
+def test_advect_break_copies_left_and_leaves_rest_of_row():
+    """fluid.rs:420-423: when the back-trace reaches the far edge the cell gets
+    its (just written) left neighbour and the rest of the row is untouched."""
+    S = 10
+    d0 = np.arange(S * S, dtype=np.float32)
+    u = np.zeros(S * S, np.float32)
+    v = np.zeros(S * S, np.float32)
+    u[5 + 3 * S] = -100.0      # row 3, column 5 triggers
+    u[1 + 6 * S] = -100.0      # row 6, column 1 triggers
+    v[4 + 7 * S] = -100.0      # row 7, column 4 triggers through y
+    d = np.full(S * S, -7.0, np.float32)
+    d[0 + 6 * S] = 123.0       # ring cell left of row 6
+    oracle.advect(PASSIVE, d, d0, u, v, S, np.float32(0.1), np.zeros(S * S, np.uint8) + FLUID)
+    g, w = d.reshape(S, S), d0.reshape(S, S)
+    assert np.array_equal(g[3, 1:5], w[3, 1:5]) and g[3, 5] == w[3, 4]
+    assert np.all(g[3, 6:9] == -7.0)
+    assert g[6, 1] == 123.0 and np.all(g[6, 2:9] == -7.0)
+    assert np.array_equal(g[7, 1:4], w[7, 1:4]) and g[7, 4] == w[7, 3] and np.all(g[7, 5:9] == -7.0)
+    assert np.array_equal(g[2, 1:9], w[2, 1:9])
+
+
+# ---------------------------------------------------------------- project --
+
+def test_project_divergence_free_field_is_unchanged():
+    """u depends on y only, v on x only: div = 0, p stays 0, the gradient step
+    subtracts nothing; only set_boundaries changes cells (fluid.rs:339-374)."""
+    S = 12
+    rng = np.random.default_rng(5)
+    uy = rng.standard_normal(S)
+    vx = rng.standard_normal(S)
+    u = np.repeat(uy, S).astype(np.float64)          # u(i,j) = uy[j]
+    v = np.tile(vx, S).astype(np.float64)            # v(i,j) = vx[i]
+    p = rng.standard_normal(S * S)
+    div = rng.standard_normal(S * S)
+    m = ring_mask(S)
+    u_want, v_want = u.copy(), v.copy()
+    oracle.set_bnd(ROW, u_want, S, m)
+    oracle.set_bnd(COL, v_want, S, m)
+    oracle.project(u, v, p, div, S, 10, m)
+    assert np.array_equal(u, u_want) and np.array_equal(v, v_want)
+    assert np.all(p.reshape(S, S)[1:-1, 1:-1] == 0.0)
+    assert np.all(div.reshape(S, S)[1:-1, 1:-1] == 0.0)
+
+
+def test_project_div_and_one_pressure_sweep_hand_case():
+    """4x4 grid, u(2,1) = 1: div(1,1) = -0.5*1/4 = -0.125 (divided by the
+    ring-inclusive size), and one sweep gives p(1,1) = -1/32,
+    p(2,1) = p(1,2) = -1/128, p(2,2) = -1/256."""
+    S = 4
+    u = np.zeros(S * S)
+    v = np.zeros(S * S)
+    u[2 + S] = 1.0
+    p = np.zeros(S * S)
+    div = np.zeros(S * S)
+    oracle.project(u, v, p, div, S, 1, ring_mask(S))
+    d, pp = div.reshape(S, S), p.reshape(S, S)
+    assert d[1, 1] == -0.125 and d[1, 2] == 0 and d[2, 1] == 0 and d[0, 1] == -0.125
+    assert pp[1, 1] == -1 / 32 and pp[1, 2] == -1 / 128 and pp[2, 1] == -1 / 128 and pp[2, 2] == -1 / 256
+    assert pp[1, 0] == -1 / 32 and pp[3, 2] == -1 / 256
+
+
+def test_project_gradient_step_on_interior_cell():
+    """u(i,j) -= 0.5*(p(i+1,j)-p(i-1,j))*S on a cell that no wall touches."""
+    S = 7
+    rng = np.random.default_rng(6)
+    u = rng.standard_normal(S * S)
+    v = rng.standard_normal(S * S)
+    u_in, v_in = u.copy(), v.copy()
+    p = np.zeros(S * S)
+    div = np.zeros(S * S)
+    oracle.project(u, v, p, div, S, 3, ring_mask(S))
+    for (i, j) in ((3, 3), (2, 3), (3, 2)):
+        q = i + j * S
+        assert u[q] == u_in[q] - 0.5 * (p[q + 1] - p[q - 1]) * S
+        assert v[q] == v_in[q] - 0.5 * (p[q + S] - p[q - S]) * S
+
+
+# ------------------------------------------------------------------- step --
+
+def test_step_with_zero_velocity_keeps_interior_density():
+    S = 16
+    st = oracle.State.new(S)
+    st.f["vx"][:] = 0
+    st.f["vy"][:] = 0
+    st.fill_rect(4, 4, 7, 9)
+    d_before = st.f["density"].reshape(S, S)[1:-1, 1:-1].copy()
+    st.run(0.1, 0.0, 0.0, 5, 3)
+    assert np.array_equal(st.f["density"].reshape(S, S)[1:-1, 1:-1], d_before)
+
+
+def test_parallel_f64_red_black_step_equals_sequential():
+    """ora_step_par_f64 (OpenMP, used for the N=4096 FP64 comparison) gives
+    the same bits as the sequential red-black f64 step."""
+    from equilibrium_amd import scenes
+    sc = scenes.stam_scene(40, scenes.derive_seed(50), iters=7, frames=1, n_rects=3)
+    a = oracle.State.new(sc["size"])
+    for r in sc["rects"]:
+        a.fill_rect(*r)
+    a = a.astype(np.float64)
+    a.f["density"] += sc["src_d"]
+    a.f["scratch"] += sc["src_d"]
+    a.f["vx"] += sc["src_u"]
+    a.f["vy"] += sc["src_v"]
+    b = a.copy()
+    for _ in range(3):
+        a.step_par_f64(0.1, 1e-4, 1e-3, 7)
+        b.run(0.1, 1e-4, 1e-3, 7, 1, order=RB)
+    for k in oracle.FIELDS:
+        assert np.array_equal(a.f[k], b.f[k]), k
+
+
+def test_lexicographic_and_red_black_steps_differ():
+    """The order is observable at step level (so the bit-exact GPU test of the
+    wavefront mode would catch a non-lexicographic update)."""
+    from equilibrium_amd import scenes
+    sc = scenes.stam_scene(24, scenes.derive_seed(51), iters=5, frames=1)
+    a = oracle.State.new(sc["size"])
+    b = oracle.State.new(sc["size"])
+    a.run(0.1, 1e-4, 1e-3, 5, 1, order=LEX, src_d=sc["src_d"])
+    b.run(0.1, 1e-4, 1e-3, 5, 1, order=RB, src_d=sc["src_d"])
+    assert not np.array_equal(a.f["density"], b.f["density"])
