@@ -86,6 +86,7 @@ def lib():
             "eqf_download": (I, [P, I, P]),
             "eqf_step": (I, [P, I64]),
             "eqf_sync": (I, [P]),
+            "eqf_scene_set_tuning": (I, [P, I, I]),
             "eqf_batch_create": (I, [P, PP, I, PP]),
             "eqf_batch_step": (I, [P, I64]),
             "eqf_batch_sync": (I, [P]),
@@ -222,6 +223,9 @@ class Scene:
 
     def sync(self):
         check(lib().eqf_sync(self.h))
+
+    def set_tuning(self, tile, sweeps_per_block):
+        check(lib().eqf_scene_set_tuning(self.h, tile, sweeps_per_block))
 
 
 class Batch:

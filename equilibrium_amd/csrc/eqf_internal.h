@@ -42,4 +42,5 @@ namespace eqf {
 void* rb_create(eqf_scene* s, std::string& err);
 void rb_destroy(void* rb);
 int rb_step(eqf_scene* s, int64_t nframes, std::string& err);
+int rb_set_tiling(eqf_scene* s, int tile, int tsweeps);
 }  // namespace eqf

@@ -1,16 +1,473 @@
-// eqf_redblack.hip -- red-black Gauss-Seidel mode (row-major layout).
+// eqf_redblack.hip -- red-black mode of the Stam frame step (row-major layout).
+//
+// Every lin_solve (fluid.rs:301-325) updates the (i+j)-even cells of a sweep
+// first and the odd cells second, then applies set_boundaries, so all cells
+// of one colour are independent and the whole grid is updated in parallel by
+// many workgroups.  Sweeps are temporally blocked in LDS: a workgroup loads
+// its TX x TY tile plus a halo of 3T cells, runs T full sweeps (red, black,
+// set_boundaries) in LDS, and writes the tile back, so x and x0 cross HBM
+// once per T sweeps instead of once per sweep.  Each full sweep shrinks the
+// valid region by 3 cells: red reads black neighbours (1), black reads the
+// new red neighbours (2), and set_boundaries copies a wall neighbour (3).
+// The tile pass reads one buffer and writes another (ping-pong).
+//
+// The other phases (divergence, gradient, advection with its row `break`,
+// set_boundaries lists, copy) are grid-wide kernels.  One frame is captured
+// once as a HIP graph and replayed.  Results are identical to the red-black
+// f32 CPU oracle bit for bit and are compared with its f64 version for
+// accuracy.  Compiled with -ffp-contract=off like the exact mode.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
 
 #include "eqf_internal.h"
 
 namespace eqf {
-void* rb_create(eqf_scene*, std::string& err) {
-  err = "red-black mode is not available in this build";
-  return nullptr;
+namespace rb {
+
+__device__ __forceinline__ float rclamp(float x, float lo, float hi) {
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  return x;
 }
-void rb_destroy(void*) {}
-int rb_step(eqf_scene*, int64_t, std::string& err) {
-  err = "red-black mode is not available in this build";
-  return EQF_EINVAL;
+__device__ __forceinline__ int sat_idx(float f, int S) {
+  if (!(f > 0.0f)) return 0;
+  if (f >= (float)(S - 1)) return S - 1;
+  return (int)f;
 }
+
+// Red-black Gauss-Seidel, T full sweeps per launch on a TX x TY tile.
+template <int MODE, int T, int TX, int TY>
+__global__ void __launch_bounds__(256) k_rb_gs(const SceneDesc* __restrict__ dp, const float* __restrict__ src,
+                                               float* __restrict__ dst, const float* __restrict__ x0g, float a,
+                                               float cr, int nsweeps) {
+  constexpr int H = 3 * T;
+  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
+  constexpr int P = RW + 1;  // row pitch in LDS
+  __shared__ float sx[RH * P];
+  __shared__ float s0[RH * P];
+  __shared__ uint8_t scode[RH * P];
+  const SceneDesc& d = *dp;
+  const int S = d.S, N = d.N;
+  const int gx0 = blockIdx.x * TX - H, gy0 = blockIdx.y * TY - H;
+  const uint8_t* __restrict__ code = d.code;
+  const uint8_t* __restrict__ rowf = d.rowf;
+  for (int e = threadIdx.x; e < RW * RH; e += 256) {
+    const int ry = e / RW, rx = e - ry * RW;
+    const int gx = gx0 + rx, gy = gy0 + ry;
+    const bool in = gx >= 0 && gx < S && gy >= 0 && gy < S;
+    const int g = gy * S + gx;
+    sx[ry * P + rx] = in ? src[g] : 0.0f;
+    s0[ry * P + rx] = in ? x0g[g] : 0.0f;
+    uint8_t c = in ? code[g] : 0;
+    // Ring cells carry their Passive copy rule in the FLUID bit: a ring cell
+    // of column i / row j is rewritten iff that column / row has a NoWall cell.
+    if (in && (gy == 0 || gy == S - 1) && gx > 0 && gx < S - 1) c = (code[(gy == 0 ? 1 : S - 2) * S + gx] & C_COLF) ? 1 : 0;
+    if (in && (gx == 0 || gx == S - 1) && gy > 0 && gy < S - 1) c = rowf[gy] ? 1 : 0;
+    scode[ry * P + rx] = c;
+  }
+  __syncthreads();
+  constexpr int HW = (RW + 1) / 2;
+  for (int k = 0; k < nsweeps; ++k) {
+    for (int color = 0; color < 2; ++color) {
+      for (int e = threadIdx.x; e < RH * HW; e += 256) {
+        const int ry = e / HW, m = e - ry * HW;
+        const int rx = ((color + gx0 + gy0 + ry) & 1) + 2 * m;
+        const int gx = gx0 + rx, gy = gy0 + ry;
+        if (rx >= 1 && rx < RW - 1 && ry >= 1 && ry < RH - 1 && gx >= 1 && gx <= N && gy >= 1 && gy <= N) {
+          const int q = ry * P + rx;
+          sx[q] = (s0[q] + a * (((sx[q + 1] + sx[q - 1]) + sx[q + P]) + sx[q - P])) * cr;  // fluid.rs:315-320
+        }
+      }
+      __syncthreads();
+    }
+    // set_boundaries phase 1 (fluid.rs:141-188): writes only NoWall or ring
+    // cells, reads only walls or interior cells, so it is order-free.
+    for (int e = threadIdx.x; e < RW * RH; e += 256) {
+      const int ry = e / RW, rx = e - ry * RW;
+      const int gx = gx0 + rx, gy = gy0 + ry;
+      if (rx < 1 || rx >= RW - 1 || ry < 1 || ry >= RH - 1 || gx < 0 || gx >= S || gy < 0 || gy >= S) continue;
+      const int q = ry * P + rx;
+      const uint32_t c = scode[q];
+      if constexpr (MODE == M_ROW) {
+        if (gx >= 1 && gx <= N && gy >= 1 && gy <= N && (c & C_FLUID)) {
+          if (c & C_RWALL) sx[q] = -sx[q + 1];
+          else if (c & C_LWALL) sx[q] = -sx[q - 1];
+        }
+      } else if constexpr (MODE == M_COL) {
+        if (gx >= 1 && gx <= N && gy >= 1 && gy <= N && (c & C_FLUID)) {
+          if (c & C_UWALL) sx[q] = -sx[q - P];
+          else if (c & C_DWALL) sx[q] = -sx[q + P];
+        }
+      } else {
+        if (c & 1) {
+          if (gy == 0 && gx >= 1 && gx <= N) sx[q] = sx[q + P];
+          else if (gy == S - 1 && gx >= 1 && gx <= N) sx[q] = sx[q - P];
+          else if (gx == 0 && gy >= 1 && gy <= N) sx[q] = sx[q + 1];
+          else if (gx == S - 1 && gy >= 1 && gy <= N) sx[q] = sx[q - 1];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // corners (fluid.rs:265-271) from the final ring values
+  if (threadIdx.x < 4 && nsweeps > 0) {
+    const int cx = (threadIdx.x & 1) ? S - 1 : 0, cy = (threadIdx.x & 2) ? S - 1 : 0;
+    const int rx = cx - gx0, ry = cy - gy0;
+    if (rx >= H && rx < H + TX && ry >= H && ry < H + TY) {
+      const int q = ry * P + rx;
+      const int hx = cx == 0 ? 1 : -1, vy = cy == 0 ? P : -P;
+      // (0,0): x(1,0)+x(0,1); (0,S-1): x(1,S-1)+x(0,S-2); (S-1,0): x(S-2,0)+x(S-1,1); (S-1,S-1): x(S-2,S-1)+x(S-1,S-2)
+      sx[q] = 0.5f * (sx[q + hx] + sx[q + vy]);
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < TX * TY; e += 256) {
+    const int ty = e / TX, tx = e - ty * TX;
+    const int gx = gx0 + H + tx, gy = gy0 + H + ty;
+    if (gx < S && gy < S) dst[gy * S + gx] = sx[(ty + H) * P + tx + H];
+  }
+}
+
+// project, first half (fluid.rs:339-349)
+__global__ void k_div(const SceneDesc* __restrict__ dp, const float* __restrict__ u, const float* __restrict__ v,
+                      float* __restrict__ div, float* __restrict__ p) {
+  const SceneDesc& d = *dp;
+  const int S = d.S, N = d.N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  if (i > N) return;
+  const int q = j * S + i;
+  div[q] = -0.5f * (u[q + 1] - u[q - 1] + v[q + S] - v[q - S]) / d.sf;
+  p[q] = 0.0f;
+}
+
+// project, second half (fluid.rs:364-371)
+__global__ void k_grad(const SceneDesc* __restrict__ dp, float* __restrict__ u, float* __restrict__ v,
+                       const float* __restrict__ p) {
+  const SceneDesc& d = *dp;
+  const int S = d.S, N = d.N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  if (i > N) return;
+  const int q = j * S + i;
+  u[q] = u[q] - 0.5f * (p[q + 1] - p[q - 1]) * d.sf;
+  v[q] = v[q] - 0.5f * (p[q + S] - p[q - S]) * d.sf;
+}
+
+struct Trace {
+  int i0, i1, j0, j1;
+  float s0, s1, t0, t1;
+  bool trig;
+};
+__device__ __forceinline__ Trace back_trace(const SceneDesc& d, int c, int j, float uu, float vv) {
+  Trace r;
+  float x = (float)c - d.dt0 * uu;
+  float y = (float)j - d.dt0 * vv;
+  x = rclamp(x, 0.5f, d.sfm1);
+  y = rclamp(y, 0.5f, d.sfm1);
+  const float i0 = floorf(x), i1 = i0 + 1.0f, j0 = floorf(y), j1 = j0 + 1.0f;
+  r.s1 = x - i0;
+  r.s0 = 1.0f - r.s1;
+  r.t1 = y - j0;
+  r.t0 = 1.0f - r.t1;
+  r.trig = (i1 >= d.sf) || (j1 >= d.sf);
+  r.i0 = sat_idx(i0, d.S);
+  r.i1 = sat_idx(i1, d.S);
+  r.j0 = sat_idx(j0, d.S);
+  r.j1 = sat_idx(j1, d.S);
+  return r;
+}
+__device__ __forceinline__ float gather(const float* __restrict__ d0, const Trace& r, int S) {
+  return r.s0 * (r.t0 * d0[r.j0 * S + r.i0] + r.t1 * d0[r.j1 * S + r.i0]) +
+         r.s1 * (r.t0 * d0[r.j0 * S + r.i1] + r.t1 * d0[r.j1 * S + r.i1]);
+}
+
+// advect, pass A: first column of each row whose back-trace reaches the far
+// edge (the `break` of fluid.rs:420-423); N+1 when none.
+__global__ void __launch_bounds__(256) k_adv_trig(const SceneDesc* __restrict__ dp, const float* __restrict__ u,
+                                                  const float* __restrict__ v, int* __restrict__ trig) {
+  const SceneDesc& d = *dp;
+  const int S = d.S, N = d.N;
+  const int j = blockIdx.x + 1;
+  int t = N + 1;
+  for (int i = threadIdx.x + 1; i <= N; i += 256) {
+    const Trace r = back_trace(d, i, j, u[j * S + i], v[j * S + i]);
+    if (r.trig) {
+      t = i;
+      break;
+    }
+  }
+  __shared__ int red[256];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = min(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) trig[j] = red[0];
+}
+
+// advect, pass B (fluid.rs:398-430) for one or two fields sharing u, v.
+__global__ void k_adv(const SceneDesc* __restrict__ dp, float* __restrict__ o0, const float* __restrict__ i0,
+                      float* __restrict__ o1, const float* __restrict__ i1, const float* __restrict__ u,
+                      const float* __restrict__ v, const int* __restrict__ trig) {
+  const SceneDesc& d = *dp;
+  const int S = d.S, N = d.N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+  if (i > N) return;
+  const int tr = trig[j];
+  if (i > tr) return;  // right of the break: buffer keeps its content
+  const int q = j * S + i;
+  if (i == tr && i == 1) {  // copies the ring cell d(0, j)
+    o0[q] = o0[q - 1];
+    if (o1) o1[q] = o1[q - 1];
+    return;
+  }
+  const int c = i == tr ? i - 1 : i;  // the trigger cell copies its left neighbour's value
+  const Trace r = back_trace(d, c, j, u[j * S + c], v[j * S + c]);
+  o0[q] = gather(i0, r, S);
+  if (o1) o1[q] = gather(i1, r, S);
+}
+
+// set_boundaries as lists (built on the host from the mask), then corners.
+__global__ void k_bnd(const SceneDesc* __restrict__ dp, int mode0, float* __restrict__ x0, int mode1,
+                      float* __restrict__ x1) {
+  const SceneDesc& d = *dp;
+  const int off[3] = {0, d.nbnd[0], d.nbnd[0] + d.nbnd[1]};
+  const int n0 = d.nbnd[mode0], n1 = x1 ? d.nbnd[mode1] : 0;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n0) {
+    const int2 p = d.bnd[off[mode0] + e];
+    x0[p.x] = (mode0 == M_PASSIVE ? 1.0f : -1.0f) * x0[p.y];
+  } else if (e < n0 + n1) {
+    const int2 p = d.bnd[off[mode1] + e - n0];
+    x1[p.x] = (mode1 == M_PASSIVE ? 1.0f : -1.0f) * x1[p.y];
+  }
+}
+__global__ void k_corners(const SceneDesc* __restrict__ dp, float* __restrict__ x0, float* __restrict__ x1) {
+  const SceneDesc& d = *dp;
+  const int S = d.S;
+  float* x = threadIdx.x == 0 ? x0 : x1;
+  if (!x) return;
+  x[0] = 0.5f * (x[1] + x[S]);
+  x[(S - 1) * S] = 0.5f * (x[1 + (S - 1) * S] + x[(S - 2) * S]);
+  x[S - 1] = 0.5f * (x[S - 2] + x[S - 1 + S]);
+  x[S * S - 1] = 0.5f * (x[S - 2 + (S - 1) * S] + x[S - 1 + (S - 2) * S]);
+}
+
+__global__ void k_sources(const SceneDesc* __restrict__ dp) {
+  const SceneDesc& d = *dp;
+  const int64_t n = d.fstride;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)d.S * d.S) return;
+  const float a = d.src[e];
+  d.fld[F_D * n + e] += a;
+  d.fld[F_SC * n + e] += a;
+  d.fld[F_U * n + e] += d.src[n + e];
+  d.fld[F_V * n + e] += d.src[2 * n + e];
+}
+
+__global__ void k_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) dst[e] = src[e];
+}
+
+}  // namespace rb
+
+// ------------------------------------------------------------------ host --
+struct RBWork {
+  float* tmp = nullptr;   // ping-pong field
+  int* trig = nullptr;    // per-row break column
+  hipGraphExec_t graph = nullptr;
+  int tile = 0;           // tile configuration index
+  int tsweeps = 4;        // sweeps per LDS block
+  uint64_t key = 0;       // parameters the graph was built for
+};
+
+struct TileCfg {
+  int tx, ty;
+};
+static const TileCfg kTiles[] = {{64, 64}, {128, 32}, {32, 32}};
+
+template <int MODE, int T, int TX, int TY>
+static void launch_gs_t(hipStream_t st, const SceneDesc* dd, const float* src, float* dst, const float* x0, float a,
+                        float cr, int nsw, int S) {
+  dim3 grid((S + TX - 1) / TX, (S + TY - 1) / TY);
+  hipLaunchKernelGGL((rb::k_rb_gs<MODE, T, TX, TY>), grid, dim3(256), 0, st, dd, src, dst, x0, a, cr, nsw);
+}
+
+template <int MODE, int TX, int TY>
+static void launch_gs_tile(hipStream_t st, int T, const SceneDesc* dd, const float* src, float* dst,
+                           const float* x0, float a, float cr, int nsw, int S) {
+  if (T == 1) launch_gs_t<MODE, 1, TX, TY>(st, dd, src, dst, x0, a, cr, nsw, S);
+  else if (T == 2) launch_gs_t<MODE, 2, TX, TY>(st, dd, src, dst, x0, a, cr, nsw, S);
+  else launch_gs_t<MODE, 4, TX, TY>(st, dd, src, dst, x0, a, cr, nsw, S);
+}
+
+template <int MODE>
+static void launch_gs_mode(hipStream_t st, int tile, int T, const SceneDesc* dd, const float* src, float* dst,
+                           const float* x0, float a, float cr, int nsw, int S) {
+  if (tile == 1) launch_gs_tile<MODE, 128, 32>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+  else if (tile == 2) launch_gs_tile<MODE, 32, 32>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+  else launch_gs_tile<MODE, 64, 64>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+}
+
+// lin_solve in red-black order: ceil(iters/T) tile passes, ping-ponging
+// between x and the scratch field; an odd pass count ends with a copy back.
+static void rb_lin_solve(eqf_scene* s, RBWork* w, int mode, float* x, const float* x0, float a, float cr,
+                         int iters) {
+  if (iters <= 0) return;
+  const int S = s->S, T = w->tsweeps;
+  const SceneDesc* dd = s->d_desc;
+  float* a_ = x;
+  float* b_ = w->tmp;
+  for (int done = 0; done < iters;) {
+    const int nsw = std::min(T, iters - done);
+    if (mode == M_ROW) launch_gs_mode<M_ROW>(s->stream, w->tile, T, dd, a_, b_, x0, a, cr, nsw, S);
+    else if (mode == M_COL) launch_gs_mode<M_COL>(s->stream, w->tile, T, dd, a_, b_, x0, a, cr, nsw, S);
+    else launch_gs_mode<M_PASSIVE>(s->stream, w->tile, T, dd, a_, b_, x0, a, cr, nsw, S);
+    std::swap(a_, b_);
+    done += nsw;
+  }
+  if (a_ != x) {
+    const int64_t n = (int64_t)S * S;
+    hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, x, a_, n);
+  }
+}
+
+static void rb_bnd(eqf_scene* s, int mode0, float* x0, int mode1, float* x1) {
+  const int n = s->nbnd[mode0] + (x1 ? s->nbnd[mode1] : 0);
+  if (n > 0)
+    hipLaunchKernelGGL(rb::k_bnd, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->d_desc, mode0, x0, mode1, x1);
+  hipLaunchKernelGGL(rb::k_corners, dim3(1), dim3(2), 0, s->stream, s->d_desc, x0, x1);
+}
+
+static void rb_frame(eqf_scene* s, RBWork* w) {
+  const int S = s->S, N = s->N;
+  const SceneDesc& d = s->desc;
+  float* f = s->d_fld;
+  const int64_t n = s->fstride;
+  float *Dn = f + F_D * n, *Sc = f + F_SC * n, *U = f + F_U * n, *V = f + F_V * n, *U0 = f + F_U0 * n,
+        *V0 = f + F_V0 * n;
+  hipStream_t st = s->stream;
+  const dim3 rows((N + 255) / 256, N);
+  if (d.has_src)
+    hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)(((int64_t)S * S + 255) / 256)), dim3(256), 0, st, s->d_desc);
+  rb_lin_solve(s, w, M_ROW, U0, U, d.a_visc, d.cr_visc, d.diff_iters);  // fluid.rs:438
+  rb_lin_solve(s, w, M_COL, V0, V, d.a_visc, d.cr_visc, d.diff_iters);  // fluid.rs:448
+  // project(U0, V0, p=U, div=V), fluid.rs:459
+  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, s->d_desc, U0, V0, V, U);
+  rb_bnd(s, M_PASSIVE, V, M_PASSIVE, U);
+  rb_lin_solve(s, w, M_PASSIVE, U, V, 1.0f, 0.25f, d.press_iters);
+  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, s->d_desc, U0, V0, U);
+  rb_bnd(s, M_ROW, U0, M_COL, V0);
+  // advect U from U0 and V from V0 with (U0, V0), fluid.rs:469-489
+  hipLaunchKernelGGL(rb::k_adv_trig, dim3(N), dim3(256), 0, st, s->d_desc, U0, V0, w->trig);
+  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, s->d_desc, U, U0, V, V0, U0, V0, w->trig);
+  rb_bnd(s, M_ROW, U, M_COL, V);
+  // project(U, V, p=U0, div=V0), fluid.rs:491
+  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, s->d_desc, U, V, V0, U0);
+  rb_bnd(s, M_PASSIVE, V0, M_PASSIVE, U0);
+  rb_lin_solve(s, w, M_PASSIVE, U0, V0, 1.0f, 0.25f, d.press_iters);
+  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, s->d_desc, U, V, U0);
+  rb_bnd(s, M_ROW, U, M_COL, V);
+  // density, fluid.rs:501-523
+  rb_lin_solve(s, w, M_PASSIVE, Sc, Dn, d.a_diff, d.cr_diff, d.diff_iters);
+  hipLaunchKernelGGL(rb::k_adv_trig, dim3(N), dim3(256), 0, st, s->d_desc, U, V, w->trig);
+  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, s->d_desc, Dn, Sc, (float*)nullptr, (const float*)nullptr,
+                     U, V, w->trig);
+  rb_bnd(s, M_PASSIVE, Dn, M_PASSIVE, nullptr);
+  const int64_t nn = (int64_t)S * S;
+  hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Sc, Dn, nn);
+}
+
+void* rb_create(eqf_scene* s, std::string& err) {
+  auto* w = new RBWork;
+  const size_t n = (size_t)s->S * s->S;
+  hipError_t e = hipMalloc(&w->tmp, sizeof(float) * n);
+  if (e == hipSuccess) e = hipMalloc(&w->trig, sizeof(int) * (size_t)s->S);
+  if (e != hipSuccess) {
+    err = std::string("red-black workspace: ") + hipGetErrorString(e);
+    rb_destroy(w);
+    return nullptr;
+  }
+  // Default tiling; the autotuner (eqf_scene_tune) may override it.
+  w->tile = 0;
+  w->tsweeps = 4;
+  if (const char* t = getenv("EQF_RB_TSWEEPS")) w->tsweeps = std::max(1, std::min(4, atoi(t)));
+  if (w->tsweeps == 3) w->tsweeps = 2;
+  if (const char* t = getenv("EQF_RB_TILE")) w->tile = std::max(0, std::min(2, atoi(t)));
+  return w;
+}
+
+void rb_destroy(void* p) {
+  auto* w = static_cast<RBWork*>(p);
+  if (!w) return;
+  if (w->graph) (void)hipGraphExecDestroy(w->graph);
+  (void)hipFree(w->tmp);
+  (void)hipFree(w->trig);
+  delete w;
+}
+
+int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {
+  auto* w = static_cast<RBWork*>(s->rb);
+  if (!w) return EQF_EINVAL;
+  if (tile < 0 || tile > 2 || (tsweeps != 1 && tsweeps != 2 && tsweeps != 4)) return EQF_EINVAL;
+  if (w->tile != tile || w->tsweeps != tsweeps) {
+    w->tile = tile;
+    w->tsweeps = tsweeps;
+    w->key = 0;
+  }
+  return EQF_OK;
+}
+
+int rb_step(eqf_scene* s, int64_t nframes, std::string& err) {
+  auto* w = static_cast<RBWork*>(s->rb);
+  // Parameters are baked into the graph: rebuild when they change.
+  const SceneDesc& d = s->desc;
+  uint64_t key = 1469598103934665603ull;
+  auto mix = [&key](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+  mix((uint64_t)d.diff_iters);
+  mix((uint64_t)d.press_iters);
+  mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_visc));
+  mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_diff));
+  mix((uint64_t)d.has_src);
+  mix((uint64_t)d.nbnd[0] | ((uint64_t)d.nbnd[1] << 21) | ((uint64_t)d.nbnd[2] << 42));
+  mix((uint64_t)w->tile * 16 + (uint64_t)w->tsweeps);
+  if (!w->graph || w->key != key) {
+    if (w->graph) (void)hipGraphExecDestroy(w->graph);
+    w->graph = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      err = std::string("capture: ") + hipGetErrorString(e);
+      return EQF_EDEVICE;
+    }
+    rb_frame(s, w);
+    hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(s->stream, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+      err = std::string("red-black frame capture: ") + hipGetErrorString(le != hipSuccess ? le : e);
+      if (g) (void)hipGraphDestroy(g);
+      return EQF_EDEVICE;
+    }
+    e = hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      err = std::string("graph instantiate: ") + hipGetErrorString(e);
+      return EQF_EDEVICE;
+    }
+    w->key = key;
+  }
+  for (int64_t f = 0; f < nframes; ++f) {
+    hipError_t e = hipGraphLaunch(w->graph, s->stream);
+    if (e != hipSuccess) {
+      err = std::string("graph launch: ") + hipGetErrorString(e);
+      return EQF_EDEVICE;
+    }
+  }
+  return EQF_OK;
+}
+
 }  // namespace eqf

@@ -589,6 +589,14 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
   return EQF_OK;
 }
 
+extern "C" int eqf_scene_set_tuning(eqf_scene* s, int tile, int sweeps_per_block) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (s->layout != L_ROWMAJOR) return fail(EQF_EINVAL, "tuning applies to red-black scenes");
+  if (rb_set_tiling(s, tile, sweeps_per_block) != EQF_OK)
+    return fail(EQF_EINVAL, "tile must be 0..2 and sweeps_per_block 1, 2 or 4");
+  return EQF_OK;
+}
+
 extern "C" int eqf_sync(eqf_scene* s) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
   HIPCHK(hipStreamSynchronize(s->stream));

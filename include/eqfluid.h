@@ -125,6 +125,11 @@ int eqf_step(eqf_scene *s, int64_t nframes);
 /* Waits for the scene's queued work. */
 int eqf_sync(eqf_scene *s);
 
+/* Red-black mode tiling (no counterpart in the reference): tile shape index
+ * (0: 64x64, 1: 128x32, 2: 32x32 cells) and Gauss-Seidel sweeps per LDS
+ * block (1, 2 or 4).  Chosen by the autotuner (equilibrium_amd/tune.py). */
+int eqf_scene_set_tuning(eqf_scene *s, int tile, int sweeps_per_block);
+
 /* Batches: many scenes (any sizes, wavefront mode) advanced by one fused
  * launch per frame; K frames are captured once as a HIP graph and replayed.
  * Scenes stay owned by the caller but must not be stepped separately while

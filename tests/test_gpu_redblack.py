@@ -1,0 +1,87 @@
+"""Red-black mode on the GPU.
+
+* Bit-for-bit equality with the red-black f32 CPU oracle for every tile shape
+  and every sweeps-per-LDS-block setting (temporal blocking must not change a
+  single bit: halo cells are recomputed with the same operations).
+* Accuracy against the red-black f64 oracle: norm-wise relative error of
+  density and velocity and the pressure Poisson residual.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from equilibrium_amd import _abi, scenes
+from helpers import FIELD_NAMES, assert_state_equal, download_all, gpu_scene, oracle_state, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(label):
+    print(f"[case] {label}", flush=True)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tsw", [1, 2, 4])
+def test_redblack_bit_exact_all_tilings(tile, tsw):
+    N = 150
+    _case(f"rb tile={tile} T={tsw} N={N}")
+    sc = scenes.stam_scene(N, scenes.derive_seed(20, N), iters=7, frames=3, n_rects=4,
+                           diffusion=1e-4, viscosity=1e-3)
+    s = gpu_scene(sc, _abi.GS_REDBLACK)
+    s.set_tuning(tile, tsw)
+    s.step(sc["frames"])
+    s.sync()
+    st = run_oracle(sc, order=oracle.RB)
+    assert_state_equal(download_all(s), st, f"rb tile={tile} T={tsw}")
+
+
+@pytest.mark.parametrize("N,iters,frames", [(8, 3, 2), (30, 5, 3), (64, 20, 2), (201, 9, 2)])
+def test_redblack_bit_exact_sizes(N, iters, frames):
+    _case(f"rb N={N}")
+    sc = scenes.stam_scene(N, scenes.derive_seed(21, N), iters=iters, frames=frames, n_rects=3)
+    s = gpu_scene(sc, _abi.GS_REDBLACK)
+    s.step(frames)
+    s.sync()
+    st = run_oracle(sc, order=oracle.RB)
+    assert_state_equal(download_all(s), st, f"rb N={N}")
+
+
+def test_redblack_break_trigger_bit_exact():
+    N = 96
+    _case("rb break")
+    sc = scenes.stam_scene(N, scenes.derive_seed(22), iters=5, frames=3, n_rects=2)
+    S = N + 2
+    u = np.full(S * S, -40.0, np.float32)
+    v = np.full(S * S, 3.0, np.float32)
+    v[(np.arange(S * S) // S) % 7 == 0] = -50.0
+    sc["init_u"], sc["init_v"] = u, v
+    s = gpu_scene(sc, _abi.GS_REDBLACK)
+    s.step(sc["frames"])
+    s.sync()
+    assert_state_equal(download_all(s), run_oracle(sc, order=oracle.RB), "rb break")
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_redblack_accuracy_vs_fp64():
+    """N=254, 40 sweeps, curl-noise velocity, 8 frames: f32 GPU vs f64 CPU."""
+    N = 254
+    _case(f"rb fp64 N={N}")
+    sc = scenes.config2(N=N)
+    sc["frames"] = 8
+    s = gpu_scene(sc, _abi.GS_REDBLACK)
+    st = oracle_state(sc).astype(np.float64)
+    worst = {}
+    for f in range(sc["frames"]):
+        s.step(1)
+        s.sync()
+        st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], 1, order=oracle.RB,
+               src_d=sc["src_d"])
+        got = download_all(s)
+        for k in ("density", "vx", "vy"):
+            worst[k] = max(worst.get(k, 0.0), rel_err(got[k], st.f[k]))
+    print("[rb-fp64] worst per-frame relative error:", worst, flush=True)
+    for k, e in worst.items():
+        assert e < 1e-5, (k, e)
