@@ -123,6 +123,119 @@ def cpu_baseline(scs, frames, threads):
     return cells / wall, wall
 
 
+def bench_redblack_4096(steps, check_frames, N=4096):
+    """Config 2: one N=4096 grid, 40 sweeps, red-black mode with LDS temporal
+    blocking; autotuned tiling; accuracy vs the f64 red-black oracle on the
+    first frames; pressure Poisson residual of the last projection."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from equilibrium_amd import tune
+
+    sc = scenes.config2(N=N)
+    t_setup = time.perf_counter()
+    h = _abi.Scene(scenes.scene_params(sc, _abi.GS_REDBLACK))
+    scenes.apply_to_scene(h, sc)
+    h.sync()
+    log(f"rb N={N}: scene ready in {time.perf_counter() - t_setup:.1f} s")
+    probe = h.clone()
+    tile, tsw, timings = tune.tune_redblack(probe, frames=1)
+    probe.close()
+    h.set_tuning(tile, tsw)
+    log(f"rb N={N}: autotuned tile {tune.TILES[tile]} x {tsw} sweeps/block: {timings}")
+    h.step(1)
+    h.sync()
+    t0 = time.perf_counter()
+    h.step(steps)
+    h.sync()
+    wall = time.perf_counter() - t0
+    rate = N * N * steps / wall
+    res = {"value": rate, "unit": "cell-updates/s", "ms_per_frame": wall * 1e3 / steps,
+           "frames_timed": steps, "tile": list(tune.TILES[tile]), "sweeps_per_lds_block": tsw,
+           "gs_sweeps": sc["iters"], "alg_bytes_per_cell_update": b_alg(sc["iters"]),
+           "roofline_frac": rate * b_alg(sc["iters"]) / (HBM_PEAK_GBS * 1e9)}
+    h.close()
+    if check_frames > 0:
+        import oracle
+        from helpers import oracle_state
+
+        g = _abi.Scene(scenes.scene_params(sc, _abi.GS_REDBLACK))
+        scenes.apply_to_scene(g, sc)
+        g.set_tuning(tile, tsw)
+        st = oracle_state(sc).astype(np.float64)
+        srcd = np.ascontiguousarray(sc["src_d"], np.float64)
+        errs = []
+        t_or = time.perf_counter()
+        for f in range(check_frames):
+            g.step(1)
+            g.sync()
+            st.f["density"] += srcd
+            st.f["scratch"] += srcd
+            st.step_par_f64(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"])
+            e = {}
+            for name, fid in (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY)):
+                got = g.download(fid).astype(np.float64)
+                want = st.f[name]
+                e[name] = {"max_abs": float(np.max(np.abs(got - want))),
+                           "rel_norm": float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))}
+            errs.append(e)
+            log(f"rb N={N} frame {f}: {e}")
+        p_gpu = g.download(_abi.VX0).astype(np.float64)
+        div_gpu = g.download(_abi.VY0).astype(np.float64)
+        mx, rr, rd = oracle.poisson_residual(p_gpu, div_gpu, N + 2)
+        mx64, rr64, rd64 = oracle.poisson_residual(st.f["vx0"], st.f["vy0"], N + 2)
+        res["vs_fp64"] = {"frames": check_frames, "per_frame": errs,
+                          "worst_rel_norm": max(max(x["rel_norm"] for x in e.values()) for e in errs),
+                          "oracle_seconds": time.perf_counter() - t_or}
+        res["poisson_residual"] = {"gpu_max_abs": mx, "gpu_rms": rr, "div_rms": rd,
+                                   "gpu_rms_over_div_rms": rr / max(rd, 1e-300),
+                                   "fp64_rms_over_div_rms": rr64 / max(rd64, 1e-300)}
+        g.close()
+    return res
+
+
+def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
+    """Config 4: Poisson arrivals of scene requests (N log-uniform in
+    [64, n_max], frames U(50,500), config-0 style sources and obstacles),
+    admitted against an HBM budget; each rate runs `seconds` of arrivals,
+    then the remaining requests are cancelled.  p50/p99 frame latency =
+    tick durations weighted by resident scenes.  To keep the client on the
+    Poisson schedule, N is rounded to one of `buckets` log-spaced sizes whose
+    scenes (sources, obstacles) are generated before the clock starts."""
+    import math
+    sizes = sorted({int(round(64 * (n_max / 64) ** (k / (buckets - 1)))) for k in range(buckets)})
+    pool = {N: scenes.stam_scene(N, scenes.derive_seed(40, N), n_rects=1) for N in sizes}
+    logs = [math.log(N) for N in sizes]
+    out = []
+    for rate in rates:
+        reqs = scenes.config4_requests(rate, seconds, n_max=n_max)
+        srv = _abi.Server(int(budget_gb * (1 << 30)))
+        t0 = time.perf_counter()
+        tickets = []
+        lag = 0.0
+        for (ta, N, frames, seed) in reqs:
+            while time.perf_counter() - t0 < ta:
+                time.sleep(0.0002)
+            lag = max(lag, time.perf_counter() - t0 - ta)
+            nb = sizes[min(range(len(sizes)), key=lambda k: abs(logs[k] - math.log(N)))]
+            tickets.append(srv.submit(pool[nb], nframes=frames))
+        t_end = time.perf_counter() - t0
+        time.sleep(0.5)
+        st = srv.stats()
+        for t in tickets:
+            srv.release(t)
+        srv.wait_idle(120000)
+        r = {"rate_per_s": rate, "requests": len(reqs), "window_s": t_end,
+             "client_max_lag_s": lag,
+             "cell_updates_per_s": st["cell_updates"] / max(st["busy_ms"] * 1e-3, 1e-9),
+             "p50_frame_ms": st["p50_frame_ms"], "p99_frame_ms": st["p99_frame_ms"],
+             "p50_queue_ms": st["p50_queue_ms"], "p99_queue_ms": st["p99_queue_ms"],
+             "scene_frames": st["scene_frames"], "ticks": st["ticks"],
+             "max_resident": st["max_resident"], "peak_gb": st["peak_bytes"] / 2 ** 30}
+        log(f"serve rate {rate}/s: {r}")
+        srv.close()
+        out.append(r)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +244,10 @@ def main():
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="headline batch only (for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--rb-frames", type=int, default=10)
+    ap.add_argument("--rb-check-frames", type=int, default=2)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-rates", type=str, default="50,500,2000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -211,6 +328,14 @@ def main():
                                          "ms_per_frame": w1 * 1e3 / nf}
         h.close()
         log(f"N=1024 single: {1024 * 1024 * nf / w1:.4g} cell-updates/s")
+
+        # config 2: N=4096 single grid, red-black mode
+        out["n4096_single_redblack"] = bench_redblack_4096(args.rb_frames, args.rb_check_frames)
+        log(f"N=4096 red-black: {out['n4096_single_redblack']['value']:.4g} cell-updates/s")
+
+        # config 4: serving with Poisson arrivals
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        out["serving"] = bench_serving(rates, args.serve_seconds)
 
         # CPU baseline: oracle on this host, bounded sample
         threads = max(1, min(16, os.cpu_count() or 1))

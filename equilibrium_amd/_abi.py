@@ -40,6 +40,33 @@ class Params(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_active", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("params", Params), ("nframes", ctypes.c_int64), ("n_rects", ctypes.c_int32),
+                ("keep_velocity", ctypes.c_int32), ("rects", ctypes.c_void_p),
+                ("src_density", ctypes.c_void_p), ("src_vx", ctypes.c_void_p),
+                ("src_vy", ctypes.c_void_p), ("init_vx", ctypes.c_void_p), ("init_vy", ctypes.c_void_p)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("reserved", ctypes.c_int32), ("frames_done", ctypes.c_int64),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("ticks", ctypes.c_int64), ("scene_frames", ctypes.c_int64),
+                ("cell_updates", ctypes.c_double), ("busy_ms", ctypes.c_double),
+                ("p50_frame_ms", ctypes.c_double), ("p99_frame_ms", ctypes.c_double),
+                ("p50_queue_ms", ctypes.c_double), ("p99_queue_ms", ctypes.c_double),
+                ("max_resident", ctypes.c_int64), ("peak_bytes", ctypes.c_uint64)]
+
+
+REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
+
 _lib = None
 
 
@@ -92,6 +119,14 @@ def lib():
             "eqf_batch_sync": (I, [P]),
             "eqf_batch_destroy": (I, [P]),
             "eqf_batch_last_ms": (I, [P, ctypes.POINTER(ctypes.c_double)]),
+            "eqf_server_create": (I, [P, ctypes.POINTER(ServerConfig), PP]),
+            "eqf_server_submit": (I, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+            "eqf_server_poll": (I, [P, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]),
+            "eqf_server_result": (I, [P, ctypes.c_uint64, I, P]),
+            "eqf_server_release": (I, [P, ctypes.c_uint64]),
+            "eqf_server_wait_idle": (I, [P, ctypes.c_double]),
+            "eqf_server_stats_get": (I, [P, ctypes.POINTER(ServerStats)]),
+            "eqf_server_destroy": (I, [P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -251,6 +286,70 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             check(lib().eqf_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server (eqf_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, max_active=0, ctx=None):
+        self.ctx = ctx or default_context()
+        cfg = ServerConfig(hbm_budget_bytes, max_active, 0)
+        h = ctypes.c_void_p()
+        check(lib().eqf_server_create(self.ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, sc, nframes=None, keep_velocity=False):
+        """Submits a scene dict (equilibrium_amd.scenes format)."""
+        from . import scenes as _sc
+        p = _sc.scene_params(sc)
+        S = sc["size"]
+        arrs = {}
+        for k in ("src_d", "src_u", "src_v", "init_u", "init_v"):
+            a = sc.get(k)
+            arrs[k] = None if a is None else np.ascontiguousarray(a, np.float32).ravel()
+        rects = np.ascontiguousarray(np.array(sc["rects"], np.int64).reshape(-1, 4))
+        req = Request(p, sc["frames"] if nframes is None else nframes, len(sc["rects"]),
+                      1 if keep_velocity else 0, rects.ctypes.data if len(sc["rects"]) else None,
+                      *(None if arrs[k] is None else arrs[k].ctypes.data
+                        for k in ("src_d", "src_u", "src_v", "init_u", "init_v")))
+        t = ctypes.c_uint64()
+        check(lib().eqf_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
+        self._keep[t.value] = S
+        return t.value
+
+    def poll(self, ticket):
+        st = RequestStatus()
+        check(lib().eqf_server_poll(self.h, ticket, ctypes.byref(st)))
+        return st
+
+    def result(self, ticket, field=DENSITY):
+        S = self._keep[ticket]
+        out = np.empty(S * S, np.float32)
+        check(lib().eqf_server_result(self.h, ticket, field, ptr(out)))
+        return out
+
+    def release(self, ticket):
+        check(lib().eqf_server_release(self.h, ticket))
+
+    def wait_idle(self, timeout_ms=-1.0):
+        check(lib().eqf_server_wait_idle(self.h, timeout_ms))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().eqf_server_stats_get(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().eqf_server_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -37,6 +37,12 @@ struct eqf_scene {
   void* rb = nullptr;       // red-black workspace (eqf_redblack.hip)
 };
 
+// Runtime services shared with the server (eqf_runtime.hip).
+int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
+hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int maxns, int nframes, hipStream_t st);
+int eqf_internal_error(int code, const std::string& msg);
+int eqf_internal_check_params(const eqf_params* p);
+
 namespace eqf {
 // Red-black mode (eqf_redblack.hip).
 void* rb_create(eqf_scene* s, std::string& err);

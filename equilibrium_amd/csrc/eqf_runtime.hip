@@ -663,15 +663,23 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
   return EQF_OK;
 }
 
-static hipError_t batch_launch(eqf_batch* b, int nframes_per_launch) {
-  const int n = (int)b->scenes.size();
-  if (b->W <= 8)
-    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 512>), dim3(n), dim3(64 * b->W), 0, b->stream, b->d_descs,
-                       nframes_per_launch);
+hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int maxns, int nframes, hipStream_t st) {
+  const int W = waves_for(maxns);
+  if (W <= 8)
+    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 512>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
   else
-    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 1024>), dim3(n), dim3(64 * b->W), 0, b->stream, b->d_descs,
-                       nframes_per_launch);
+    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 1024>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
   return hipGetLastError();
+}
+
+int eqf_internal_prepare(eqf_scene* s) { return sync_desc(s); }
+int eqf_internal_error(int code, const std::string& msg) { return fail(code, msg); }
+int eqf_internal_check_params(const eqf_params* p) { return check_params(p); }
+
+static hipError_t batch_launch(eqf_batch* b, int nframes_per_launch) {
+  int maxns = 1;
+  for (auto* s : b->scenes) maxns = std::max(maxns, s->ns);
+  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), maxns, nframes_per_launch, b->stream);
 }
 
 extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {

@@ -141,6 +141,66 @@ int eqf_batch_destroy(eqf_batch *b);
 /* Wall time of the last eqf_batch_step in milliseconds (HIP events). */
 int eqf_batch_last_ms(const eqf_batch *b, double *ms);
 
+/* Serving: continuous batching of scene requests (no counterpart in the
+ * reference, whose frame loop runs one scene; renderer_helpers.rs:52-72).
+ * A server thread admits queued requests while their device memory fits the
+ * budget, keeps admitted scenes resident, and advances every resident scene
+ * by one frame per tick with one fused launch; scenes join and leave at tick
+ * boundaries.  Admission uploads overlap the running tick. */
+typedef struct eqf_server_config {
+    uint64_t hbm_budget_bytes; /* device memory the resident scenes may use */
+    int32_t max_active;        /* resident scenes at most (0: no limit) */
+    int32_t reserved;
+} eqf_server_config;
+
+typedef struct eqf_request {
+    eqf_params params;         /* size, delta_t, diffusion, viscosity, gs_iters; gs_mode must be wavefront */
+    int64_t nframes;           /* frames to advance */
+    int32_t n_rects;           /* obstacles: n_rects x (x0, y0, x1, y1), fill_obstacle semantics */
+    int32_t keep_velocity;     /* 1: also keep vx, vy for eqf_server_result */
+    const int64_t *rects;
+    const float *src_density;  /* per-frame sources (size*size, row-major) or NULL */
+    const float *src_vx;
+    const float *src_vy;
+    const float *init_vx;      /* initial velocity replacing Fluid::new's, or NULL */
+    const float *init_vy;
+} eqf_request;
+
+enum { EQF_REQ_QUEUED = 0, EQF_REQ_ACTIVE = 1, EQF_REQ_DONE = 2, EQF_REQ_FAILED = 3, EQF_REQ_CANCELLED = 4 };
+
+typedef struct eqf_request_status {
+    int32_t state;
+    int32_t reserved;
+    int64_t frames_done;
+    double submit_ms, admit_ms, done_ms; /* since server start */
+} eqf_request_status;
+
+typedef struct eqf_server_stats {
+    int64_t ticks;
+    int64_t scene_frames;
+    double cell_updates;      /* sum over resident scenes and ticks of N^2 */
+    double busy_ms;           /* wall time with at least one resident scene */
+    double p50_frame_ms;      /* per scene-frame latency percentiles (tick durations) */
+    double p99_frame_ms;
+    double p50_queue_ms;      /* submit -> admit */
+    double p99_queue_ms;
+    int64_t max_resident;
+    uint64_t peak_bytes;
+} eqf_server_stats;
+
+int eqf_server_create(eqf_ctx *ctx, const eqf_server_config *cfg, eqf_server **out);
+int eqf_server_submit(eqf_server *sv, const eqf_request *req, uint64_t *ticket);
+int eqf_server_poll(eqf_server *sv, uint64_t ticket, eqf_request_status *st);
+/* Copies a finished request's field (EQF_DENSITY, or EQF_VX/EQF_VY when
+ * keep_velocity was set) into `out` (size*size floats). */
+int eqf_server_result(eqf_server *sv, uint64_t ticket, int field, float *out);
+/* Forgets a ticket; cancels it if it is queued or resident. */
+int eqf_server_release(eqf_server *sv, uint64_t ticket);
+/* Waits until no request is queued or resident (timeout_ms < 0: forever). */
+int eqf_server_wait_idle(eqf_server *sv, double timeout_ms);
+int eqf_server_stats_get(eqf_server *sv, eqf_server_stats *st);
+int eqf_server_destroy(eqf_server *sv);
+
 #ifdef __cplusplus
 }
 #endif

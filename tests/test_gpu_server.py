@@ -1,0 +1,97 @@
+"""Serving loop on the GPU: continuous batching with an HBM budget.
+
+Each result must be the same alone and in the mix (bit-exact, it is the exact
+wavefront mode), requests that arrive while others are resident are admitted
+between ticks, and a cancelled scene does not disturb the others."""
+import time
+
+import numpy as np
+import pytest
+
+from equilibrium_amd import _abi, scenes
+from helpers import assert_bit_equal, gpu_scene, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(label):
+    print(f"[case] {label}", flush=True)
+
+
+def _solo(sc):
+    s = gpu_scene(sc)
+    s.step(sc["frames"])
+    s.sync()
+    d = s.download(_abi.DENSITY)
+    u = s.download(_abi.VX)
+    s.close()
+    return d, u
+
+
+def _mix():
+    out = []
+    for i, (N, fr) in enumerate([(64, 7), (200, 3), (30, 9), (128, 5), (96, 4), (150, 6), (8, 5), (70, 8)]):
+        out.append(scenes.stam_scene(N, scenes.derive_seed(30, i), iters=4, frames=fr, n_rects=i % 4,
+                                     viscosity=10 ** (-3 - i % 3), diffusion=10 ** (-5 + i % 2)))
+    return out
+
+
+def test_server_results_equal_solo_runs():
+    _case("server mix")
+    mix = _mix()
+    srv = _abi.Server(hbm_budget_bytes=8 << 30)
+    tickets = []
+    for i, sc in enumerate(mix):
+        tickets.append(srv.submit(sc, keep_velocity=True))
+        if i % 3 == 2:
+            time.sleep(0.002)  # later arrivals join a running batch
+    srv.wait_idle(60000)
+    st = srv.stats()
+    for sc, t in zip(mix, tickets):
+        s = srv.poll(t)
+        assert s.state == _abi.REQ_DONE and s.frames_done == sc["frames"]
+        d, u = _solo(sc)
+        assert_bit_equal(srv.result(t, _abi.DENSITY), d, f"density N={sc['size'] - 2}")
+        assert_bit_equal(srv.result(t, _abi.VX), u, f"vx N={sc['size'] - 2}")
+    o = run_oracle(mix[1])
+    assert_bit_equal(srv.result(tickets[1]), o.f["density"], "vs oracle")
+    assert st["scene_frames"] == sum(sc["frames"] for sc in mix)
+    assert st["p99_frame_ms"] >= st["p50_frame_ms"] > 0
+    srv.close()
+
+
+def test_server_budget_limits_residency():
+    _case("server budget")
+    mix = _mix()
+    # room for roughly one mid-size scene at a time
+    srv = _abi.Server(hbm_budget_bytes=3 << 20)
+    tickets = [srv.submit(sc) for sc in mix if sc["size"] <= 132]
+    small = [sc for sc in mix if sc["size"] <= 132]
+    srv.wait_idle(60000)
+    st = srv.stats()
+    assert st["peak_bytes"] <= 3 << 20
+    for sc, t in zip(small, tickets):
+        assert srv.poll(t).state == _abi.REQ_DONE
+        assert_bit_equal(srv.result(t), _solo(sc)[0], f"budget N={sc['size'] - 2}")
+    srv.close()
+
+
+def test_server_rejects_oversized_and_cancels():
+    _case("server cancel")
+    mix = _mix()
+    srv = _abi.Server(hbm_budget_bytes=1 << 20)
+    big = scenes.stam_scene(600, scenes.derive_seed(31), iters=2, frames=2)
+    with pytest.raises(_abi.EqfError):
+        srv.submit(big)
+    srv.close()
+    srv = _abi.Server(hbm_budget_bytes=8 << 30)
+    long_sc = scenes.stam_scene(128, scenes.derive_seed(32), iters=4, frames=10 ** 6)
+    t_long = srv.submit(long_sc)
+    ts = [srv.submit(sc) for sc in mix[:4]]
+    time.sleep(0.2)
+    srv.release(t_long)
+    srv.wait_idle(60000)
+    assert srv.poll(t_long).state == _abi.REQ_CANCELLED
+    for sc, t in zip(mix[:4], ts):
+        assert_bit_equal(srv.result(t), _solo(sc)[0], f"cancel N={sc['size'] - 2}")
+    srv.close()
