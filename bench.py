@@ -40,6 +40,15 @@ def b_alg(iters):
     return 60 * iters + 108
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch measured by the committed rocprofv3 PMC runs."""
+    try:
+        with open(os.path.join(REPO, "profiles", "traffic.json")) as f:
+            return json.load(f)[kernel]["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -243,6 +252,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="headline batch only (for profiling)")
+    ap.add_argument("--legs", type=str, default="batch,single,n1024,rb4096,serve,cpu",
+                    help="comma list of legs to run")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy first (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rb-frames", type=int, default=10)
     ap.add_argument("--rb-check-frames", type=int, default=2)
@@ -254,54 +267,72 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    legs = set(args.legs.split(","))
+    if args.no_extras:
+        legs = {"batch"}
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 1 GiB read + 1 GiB write (torch copy kernel)")
     mix = scenes.config3(n=args.scenes, frames=args.steps)
-    iters = mix[0]["iters"]
-    cells = sum((sc["size"] - 2) ** 2 for sc in mix)
-    log(f"batched mix: {len(mix)} scenes, {cells} interior cells, {iters} sweeps")
-    wall, ev_ms = bench_batch(mix, args.steps, args.warmup)
-    value = cells * args.steps / wall
-    ms_per_step = wall * 1e3 / args.steps
-    launch_ms = ev_ms / args.steps  # HIP events on the batch stream around the K launches
-    alg_bytes = cells * b_alg(iters)
-    achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
-    log(f"batch: {value:.4g} cell-updates/s, {ms_per_step:.3f} ms/frame, kernel {launch_ms:.3f} ms")
+    out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded scenes generated on the host)"}
+    value = None
+    if "batch" in legs:
+        iters = mix[0]["iters"]
+        cells = sum((sc["size"] - 2) ** 2 for sc in mix)
+        log(f"batched mix: {len(mix)} scenes, {cells} interior cells, {iters} sweeps")
+        wall, ev_ms = bench_batch(mix, args.steps, args.warmup)
+        value = cells * args.steps / wall
+        ms_per_step = wall * 1e3 / args.steps
+        launch_ms = ev_ms / args.steps  # HIP events on the batch stream around the K launches
+        alg_bytes = cells * b_alg(iters)
+        achieved = alg_bytes / (launch_ms * 1e-3) / 1e9
+        log(f"batch: {value:.4g} cell-updates/s, {ms_per_step:.3f} ms/frame, kernel {launch_ms:.3f} ms")
 
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded scenes generated on the host)",
-        "config": {
-            "workload": f"batched mix: {len(mix)} scenes, N uniform in {{64,128,256,512}}, "
-                        f"visc/diff log-U[1e-6,1e-3], 0-4 rectangles, {iters} GS sweeps, dt 0.1, "
-                        "exact wavefront mode, one fused launch per frame (HIP graph)",
-            "scenes": len(mix),
-            "interior_cells": cells,
-            "gs_sweeps": iters,
-            "mode": "wavefront (bit-exact vs CPU oracle)",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "eqf_frames_cyclic (fused frame step, one workgroup per scene)",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "alg_bytes_per_launch": alg_bytes,
-            "launch_ms": launch_ms,
-        },
-    }
+        out.update({
+            "metric": METRIC,
+            "value": value,
+            "unit": "cell-updates/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": ms_per_step,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f32",
+            "data": "synthetic (seeded scenes generated on the host)",
+            "config": {
+                "workload": f"batched mix: {len(mix)} scenes, N uniform in {{64,128,256,512}}, "
+                            f"visc/diff log-U[1e-6,1e-3], 0-4 rectangles, {iters} GS sweeps, dt 0.1, "
+                            "exact wavefront mode, one fused launch per frame (HIP graph)",
+                "scenes": len(mix),
+                "interior_cells": cells,
+                "gs_sweeps": iters,
+                "mode": "wavefront (bit-exact vs CPU oracle)",
+            },
+            "roofline": {
+                "bound": "hbm",
+                "kernel": "eqf_frames_cyclic (fused frame step, one workgroup per scene)",
+                "achieved": achieved,
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": profiled_traffic("eqf_frames_cyclic"),
+                "traffic_source": "profiles/eqf_frames_cyclic_batched_mix.md (rocprofv3 --pmc FETCH_SIZE, "
+                                  "WRITE_SIZE, scaled by a 1 GiB copy)",
+                "alg_bytes_per_launch": alg_bytes,
+                "launch_ms": launch_ms,
+            },
+        })
 
-    if not args.no_extras:
+    if "single" in legs:
         # single-grid drop-in on a stratified sample of the same mix
         sample = mix[::8]
         sframes = 2
@@ -311,9 +342,11 @@ def main():
         out["single_grid_dropin"] = {"value": single, "unit": "cell-updates/s",
                                      "sample": f"{len(sample)} scenes of the mix x {sframes} frames, "
                                                "one eqf_step per scene"}
-        out["batched_over_single"] = value / single
-        log(f"single drop-in: {single:.4g} cell-updates/s; batched/single = {value / single:.1f}x")
+        if value:
+            out["batched_over_single"] = value / single
+        log(f"single drop-in: {single:.4g} cell-updates/s")
 
+    if "n1024" in legs:
         # N=1024 single grid, exact mode
         c1 = scenes.config1()
         h = make_scenes([c1])[0]
@@ -329,27 +362,31 @@ def main():
         h.close()
         log(f"N=1024 single: {1024 * 1024 * nf / w1:.4g} cell-updates/s")
 
+    if "rb4096" in legs:
         # config 2: N=4096 single grid, red-black mode
         out["n4096_single_redblack"] = bench_redblack_4096(args.rb_frames, args.rb_check_frames)
         log(f"N=4096 red-black: {out['n4096_single_redblack']['value']:.4g} cell-updates/s")
 
+    if "serve" in legs:
         # config 4: serving with Poisson arrivals
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = bench_serving(rates, args.serve_seconds)
 
+    if "cpu" in legs:
         # CPU baseline: oracle on this host, bounded sample
         threads = max(1, min(16, os.cpu_count() or 1))
         probe = mix[:threads]
-        rate_probe, wprobe = cpu_baseline(probe, 1, threads)
-        # size the sample for about cpu_seconds of work
-        per_scene_s = wprobe / max(1, len(probe)) * threads
-        n_cpu = int(max(threads, min(len(mix), args.cpu_seconds / max(per_scene_s, 1e-3) * threads)))
-        cpu_sample = mix[: n_cpu]
-        rate, wcpu = cpu_baseline(cpu_sample, 1, threads)
+        _, wprobe = cpu_baseline(probe, 1, threads)
+        cells_probe = sum((sc["size"] - 2) ** 2 for sc in probe)
+        cells = sum((sc["size"] - 2) ** 2 for sc in mix)
+        # whole mix, as many frames as fit in about cpu_seconds
+        est_frame_s = wprobe * cells / max(cells_probe, 1)
+        cframes = int(max(1, min(20, round(args.cpu_seconds / max(est_frame_s, 1e-3)))))
+        rate, wcpu = cpu_baseline(mix, cframes, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"first {len(cpu_sample)} scenes of the mix x 1 frame, "
-                                         f"one scene per thread, {wcpu:.1f} s"}
+                               "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) from the "
+                                         f"initial state, one scene per thread, {wcpu:.1f} s"}
         log(f"cpu oracle: {rate:.4g} cell-updates/s on {threads} threads ({wcpu:.1f} s)")
 
     print(json.dumps(out), flush=True)

@@ -12,7 +12,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
-SOURCES = ["eqf_runtime.hip", "eqf_wavefront.hip", "eqf_redblack.hip", "eqf_server.hip"]
+SOURCES = ["eqf_runtime.hip", "eqf_wavefront.hip", "eqf_redblack.hip", "eqf_server.hip", "eqf_setup.hip"]
 OUT = os.path.join(PKG_DIR, "libeqfluid.so")
 ARCH = os.environ.get("EQF_OFFLOAD_ARCH", "gfx950")
 
@@ -37,6 +37,25 @@ def build(verbose=False):
         print(err, file=sys.stderr)
     os.replace(OUT + ".tmp", OUT)
     return OUT
+
+
+def build_cpp_tests(verbose=False):
+    """g++ build of tests/cpp/test_fluid_api (host code only) against
+    libeqfluid.so and the oracle library."""
+    src = os.path.join(REPO_DIR, "tests", "cpp", "test_fluid_api.cpp")
+    outdir = os.path.join(REPO_DIR, "tests", "cpp", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "test_fluid_api")
+    cmd = ["g++", "-std=c++17", "-O1", "-ffp-contract=off", src, "-o", out,
+           "-L" + PKG_DIR, "-leqfluid", "-L" + os.path.join(REPO_DIR, "oracle"), "-l:liboracle.so",
+           "-Wl,-rpath,$ORIGIN/../../../equilibrium_amd:$ORIGIN/../../../oracle:/opt/rocm/lib",
+           "-L/opt/rocm/lib", "-lamdhip64"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
+    return out
 
 
 if __name__ == "__main__":

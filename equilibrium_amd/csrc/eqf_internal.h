@@ -30,10 +30,14 @@ struct eqf_scene {
   size_t bnd_cap = 0;
   int nbnd[3] = {0, 0, 0};
   float* d_hraw = nullptr;
+  void* d_main = nullptr;     // one allocation: fields, codes, rowf, hand-off rows, descriptor
+  float* d_stage = nullptr;   // row-major staging for uploads/downloads (lazy)
+  uint8_t* d_mask = nullptr;  // mask + scratch for the boundary build (lazy)
   eqf::SceneDesc* d_desc = nullptr;
   eqf::SceneDesc desc{};
   std::vector<float> hbuf;  // layout staging
   int in_batch = 0;
+  bool owns_stream = true;
   void* rb = nullptr;       // red-black workspace (eqf_redblack.hip)
 };
 
@@ -42,6 +46,15 @@ int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
 hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int maxns, int nframes, hipStream_t st);
 int eqf_internal_error(int code, const std::string& msg);
 int eqf_internal_check_params(const eqf_params* p);
+// eqf_scene_create on a caller-owned stream (the server's admission stream).
+int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, eqf_scene** out);
+
+// Device-side setup (eqf_setup.hip).
+int eqf_dev_init_fields(eqf_scene* s, int times);
+int eqf_dev_upload(eqf_scene* s, float* dst, float* dst2, const float* host, int add);
+int eqf_dev_download(eqf_scene* s, const float* src, float* host);
+int eqf_dev_rebuild_bnd(eqf_scene* s);
+void eqf_dev_trim(eqf_scene* s);
 
 namespace eqf {
 // Red-black mode (eqf_redblack.hip).

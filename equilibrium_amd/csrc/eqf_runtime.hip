@@ -61,36 +61,13 @@ int64_t eqf_scene_off(const eqf_scene* s, int x, int y) {
 
 static float* field_ptr(const eqf_scene* s, int f) { return s->d_fld + (int64_t)f * s->fstride; }
 
-// row-major host array -> device field
+// row-major host array <-> device field (layout conversion on the device)
 static int upload_rm(eqf_scene* s, float* dfield, const float* host) {
-  const int S = s->S;
-  if (s->layout == L_ROWMAJOR) {
-    HIPCHK(hipMemcpyAsync(dfield, host, sizeof(float) * (size_t)S * S, hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return EQF_OK;
-  }
-  s->hbuf.assign(s->fstride, 0.0f);
-  for (int y = 0; y < S; ++y)
-    for (int x = 0; x < S; ++x) s->hbuf[eqf_scene_off(s, x, y)] = host[(size_t)x + (size_t)y * S];
-  HIPCHK(hipMemcpyAsync(dfield, s->hbuf.data(), sizeof(float) * s->fstride, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  return EQF_OK;
+  return eqf_dev_upload(s, dfield, nullptr, host, 0);
 }
 
 static int download_rm(const eqf_scene* cs, const float* dfield, float* host) {
-  eqf_scene* s = const_cast<eqf_scene*>(cs);
-  const int S = s->S;
-  if (s->layout == L_ROWMAJOR) {
-    HIPCHK(hipMemcpyAsync(host, dfield, sizeof(float) * (size_t)S * S, hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    return EQF_OK;
-  }
-  s->hbuf.resize(s->fstride);
-  HIPCHK(hipMemcpyAsync(s->hbuf.data(), dfield, sizeof(float) * s->fstride, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  for (int y = 0; y < S; ++y)
-    for (int x = 0; x < S; ++x) host[(size_t)x + (size_t)y * S] = s->hbuf[eqf_scene_off(s, x, y)];
-  return EQF_OK;
+  return eqf_dev_download(const_cast<eqf_scene*>(cs), dfield, host);
 }
 
 // Reference initial state (fluid.rs:93-110, 527-570, 602-606), on the host.
@@ -133,78 +110,9 @@ static void host_init(int S, int times, std::vector<std::vector<float>>& F, std:
   }
 }
 
-// Boundary codes and set_boundaries lists derived from the mask.
-static int rebuild_bnd(eqf_scene* s) {
-  const int S = s->S, N = s->N;
-  const auto& m = s->mask;
-  auto wall = [&](int x, int y) { return m[(size_t)x + (size_t)y * S] == EQF_DEFAULT_WALL; };
-  std::vector<uint8_t> colf(S, 0), rowf(S, 0);
-  for (int y = 0; y < S; ++y)
-    for (int x = 0; x < S; ++x)
-      if (!wall(x, y)) {
-        colf[x] = 1;
-        rowf[y] = 1;
-      }
-  const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * N * 64 : (size_t)S * S;
-  std::vector<uint8_t> code(ncode, 0);
-  std::vector<int2> lrow, lcol, lpas;
-  auto off = [&](int x, int y) { return (int)eqf_scene_off(s, x, y); };
-  for (int y = 1; y <= N; ++y)
-    for (int x = 1; x <= N; ++x) {
-      uint8_t c = 0;
-      const bool fl = !wall(x, y);
-      if (fl) c |= C_FLUID;
-      if (wall(x - 1, y)) c |= C_LWALL;
-      if (wall(x + 1, y)) c |= C_RWALL;
-      if (wall(x, y + 1)) c |= C_DWALL;
-      if (wall(x, y - 1)) c |= C_UWALL;
-      if (colf[x]) c |= C_COLF;
-      if (s->layout == L_CYCLIC && ((y - 1) & 63) == 0 && y > 1) {
-        // (x, y-1) is the last row of the strip above
-        if (!wall(x, y - 1) && wall(x, y) && !wall(x, y - 2)) c |= C_FIXUP;
-      }
-      code[s->layout == L_CYCLIC ? (size_t)cyc_off(x, y, S, N, s->ns) : (size_t)x + (size_t)y * S] = c;
-      if (fl) {
-        // AdjustRow: left then right, right wins (fluid.rs:151-164)
-        if (wall(x + 1, y)) lrow.push_back({off(x, y), off(x + 1, y)});
-        else if (wall(x - 1, y)) lrow.push_back({off(x, y), off(x - 1, y)});
-        // AdjustColumn: down (j+1) then up (j-1), up wins (fluid.rs:165-177)
-        if (wall(x, y - 1)) lcol.push_back({off(x, y), off(x, y - 1)});
-        else if (wall(x, y + 1)) lcol.push_back({off(x, y), off(x, y + 1)});
-      }
-    }
-  // Passive: ring copies for every column / row holding a NoWall cell (fluid.rs:179-187)
-  for (int x = 1; x <= N; ++x)
-    if (colf[x]) {
-      lpas.push_back({off(x, 0), off(x, 1)});
-      lpas.push_back({off(x, S - 1), off(x, S - 2)});
-    }
-  for (int y = 1; y <= N; ++y)
-    if (rowf[y]) {
-      lpas.push_back({off(0, y), off(1, y)});
-      lpas.push_back({off(S - 1, y), off(S - 2, y)});
-    }
-  std::vector<int2> all;
-  all.insert(all.end(), lrow.begin(), lrow.end());
-  all.insert(all.end(), lcol.begin(), lcol.end());
-  all.insert(all.end(), lpas.begin(), lpas.end());
-  s->nbnd[0] = (int)lrow.size();
-  s->nbnd[1] = (int)lcol.size();
-  s->nbnd[2] = (int)lpas.size();
-  if (all.empty()) all.push_back({0, 0});
-  if (all.size() > s->bnd_cap) {
-    if (s->d_bnd) HIPCHK(hipFree(s->d_bnd));
-    s->d_bnd = nullptr;
-    HIPCHK(hipMalloc(&s->d_bnd, sizeof(int2) * all.size()));
-    s->bnd_cap = all.size();
-  }
-  HIPCHK(hipMemcpyAsync(s->d_bnd, all.data(), sizeof(int2) * all.size(), hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(s->d_code, code.data(), ncode, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipMemcpyAsync(s->d_rowf, rowf.data(), S, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  s->bnd_dirty = false;
-  return EQF_OK;
-}
+// Boundary codes and set_boundaries lists derived from the mask (built on
+// the device from the host copy of the mask).
+static int rebuild_bnd(eqf_scene* s) { return eqf_dev_rebuild_bnd(s); }
 
 static void fill_desc(eqf_scene* s) {
   SceneDesc& d = s->desc;
@@ -252,7 +160,7 @@ static int check_params(const eqf_params* p) {
   if (p->size > 32768) return fail(EQF_EINVAL, "size must be <= 32768");
   if (p->gs_mode != EQF_GS_WAVEFRONT && p->gs_mode != EQF_GS_REDBLACK)
     return fail(EQF_EINVAL, "gs_mode must be EQF_GS_WAVEFRONT or EQF_GS_REDBLACK");
-  if (p->init_times < 0 || p->init_times > 2) return fail(EQF_EINVAL, "init_times must be 0, 1 or 2");
+  if (p->init_times < 1 || p->init_times > 2) return fail(EQF_EINVAL, "init_times must be 1 (Fluid::new) or 2 (Fluid::default)");
   const int64_t it = resolved_gs(*p), pi = resolved_press(*p);
   if (it > (1 << 24) || pi > (1 << 24)) return fail(EQF_EINVAL, "iteration count too large");
   if (p->gs_mode == EQF_GS_WAVEFRONT) {
@@ -287,18 +195,16 @@ extern "C" int eqf_ctx_destroy(eqf_ctx* ctx) {
 static void scene_free(eqf_scene* s) {
   if (!s) return;
   if (s->rb) rb_destroy(s->rb);
-  (void)hipFree(s->d_fld);
+  (void)hipFree(s->d_main);
   (void)hipFree(s->d_src);
-  (void)hipFree(s->d_code);
-  (void)hipFree(s->d_rowf);
   (void)hipFree(s->d_bnd);
-  (void)hipFree(s->d_hraw);
-  (void)hipFree(s->d_desc);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  (void)hipFree(s->d_stage);
+  (void)hipFree(s->d_mask);
+  if (s->stream && s->owns_stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
-static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
+static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipStream_t stream = nullptr) {
   int rc = check_params(p);
   if (rc) return rc;
   HIPCHK(hipSetDevice(ctx->device));
@@ -311,18 +217,32 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
   s->ns = (s->N + 63) / 64;
   s->layout = p->gs_mode == EQF_GS_WAVEFRONT ? L_CYCLIC : L_ROWMAJOR;
   s->fstride = s->layout == L_CYCLIC ? cyc_field_floats(s->S, s->N, s->ns) : rm_field_floats(s->S);
-  hipError_t e = hipStreamCreate(&s->stream);
+  hipError_t e = hipSuccess;
+  if (stream) {
+    s->stream = stream;
+    s->owns_stream = false;
+  } else {
+    e = hipStreamCreate(&s->stream);
+  }
   if (e != hipSuccess) {
     scene_free(s);
     return fail(EQF_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->N * 64 : (size_t)s->S * s->S;
-  e = hipMalloc(&s->d_fld, sizeof(float) * NFIELDS * s->fstride);
-  if (e == hipSuccess) e = hipMalloc(&s->d_code, ncode);
-  if (e == hipSuccess) e = hipMalloc(&s->d_rowf, s->S);
-  if (e == hipSuccess) e = hipMalloc(&s->d_hraw, sizeof(float) * (size_t)s->ns * s->N + 4);
-  if (e == hipSuccess) e = hipMalloc(&s->d_desc, sizeof(SceneDesc));
-  if (e == hipSuccess) e = hipMemsetAsync(s->d_fld, 0, sizeof(float) * NFIELDS * s->fstride, s->stream);
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t b_fld = up(sizeof(float) * NFIELDS * (size_t)s->fstride);
+  const size_t b_code = up(ncode), b_rowf = up((size_t)s->S);
+  const size_t b_hraw = up(sizeof(float) * ((size_t)s->ns * s->N + 4)), b_desc = up(sizeof(SceneDesc));
+  e = hipMalloc(&s->d_main, b_fld + b_code + b_rowf + b_hraw + b_desc);
+  if (e == hipSuccess) {
+    char* base = (char*)s->d_main;
+    s->d_fld = (float*)base;
+    s->d_code = (uint8_t*)(base + b_fld);
+    s->d_rowf = (uint8_t*)(base + b_fld + b_code);
+    s->d_hraw = (float*)(base + b_fld + b_code + b_rowf);
+    s->d_desc = (SceneDesc*)(base + b_fld + b_code + b_rowf + b_hraw);
+    e = hipMemsetAsync(s->d_fld, 0, sizeof(float) * NFIELDS * s->fstride, s->stream);
+  }
   if (e != hipSuccess) {
     scene_free(s);
     return fail(e == hipErrorOutOfMemory ? EQF_ENOMEM : EQF_EDEVICE,
@@ -333,14 +253,30 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
 }
 
 extern "C" int eqf_scene_create(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
+  return eqf_internal_scene_create(ctx, p, nullptr, out);
+}
+
+int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, eqf_scene** out) {
   if (!ctx || !out) return fail(EQF_EINVAL, "ctx/out is NULL");
   eqf_scene* s = nullptr;
-  int rc = scene_alloc(ctx, p, &s);
+  int rc = scene_alloc(ctx, p, &s, stream);
   if (rc) return rc;
-  std::vector<std::vector<float>> F;
-  host_init(s->S, p->init_times, F, s->mask);
-  if (p->init_times == 0) s->mask.assign((size_t)s->S * s->S, EQF_NO_WALL);
-  for (int f = 0; f < NFIELDS && !rc; ++f) rc = upload_rm(s, field_ptr(s, f), F[f].data());
+  if (s->S >= 22) {
+    // No index clamping in init_density at this size: write the state on the
+    // device; the host keeps only the mask (ring walls, init_walls).
+    const int S = s->S;
+    s->mask.assign((size_t)S * S, EQF_NO_WALL);
+    for (int i = 0; i < S; ++i) {
+      s->mask[i] = s->mask[(size_t)i + (size_t)(S - 1) * S] = EQF_DEFAULT_WALL;
+      s->mask[(size_t)i * S] = s->mask[(size_t)(S - 1) + (size_t)i * S] = EQF_DEFAULT_WALL;
+    }
+    rc = eqf_dev_init_fields(s, p->init_times);
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(EQF_EDEVICE, "scene init failed");
+  } else {
+    std::vector<std::vector<float>> F;
+    host_init(s->S, p->init_times, F, s->mask);
+    for (int f = 0; f < NFIELDS && !rc; ++f) rc = upload_rm(s, field_ptr(s, f), F[f].data());
+  }
   if (!rc && s->layout == L_ROWMAJOR) {
     std::string err;
     s->rb = rb_create(s, err);
@@ -493,36 +429,11 @@ extern "C" int eqf_add_velocity(eqf_scene* s, uint32_t x, uint32_t y, float ax, 
   return EQF_OK;
 }
 
-// Converts a row-major host array to the scene layout in a device staging buffer.
-static int stage_source(eqf_scene* s, const float* amounts, float** dstage) {
-  const int S = s->S;
-  s->hbuf.assign(s->fstride, 0.0f);
-  for (int y = 0; y < S; ++y)
-    for (int x = 0; x < S; ++x) s->hbuf[eqf_scene_off(s, x, y)] = amounts[(size_t)x + (size_t)y * S];
-  HIPCHK(hipMalloc(dstage, sizeof(float) * s->fstride));
-  HIPCHK(hipMemcpyAsync(*dstage, s->hbuf.data(), sizeof(float) * s->fstride, hipMemcpyHostToDevice, s->stream));
-  return EQF_OK;
-}
-
 extern "C" int eqf_add_source(eqf_scene* s, int field, const float* amounts) {
   if (!s || !amounts) return fail(EQF_EINVAL, "NULL argument");
   if (field != EQF_DENSITY && field != EQF_VX && field != EQF_VY)
     return fail(EQF_EINVAL, "sources exist for EQF_DENSITY, EQF_VX and EQF_VY");
-  float* st = nullptr;
-  int rc = stage_source(s, amounts, &st);
-  if (rc) {
-    (void)hipFree(st);
-    return rc;
-  }
-  float* d0 = field_ptr(s, field);
-  float* d1 = field == EQF_DENSITY ? field_ptr(s, F_SC) : nullptr;
-  const int64_t n = s->fstride;
-  hipLaunchKernelGGL(eqf_add_array_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, d0, d1, st,
-                     n);
-  hipError_t e = hipStreamSynchronize(s->stream);
-  (void)hipFree(st);
-  if (e != hipSuccess) return fail(EQF_EDEVICE, std::string("add_source: ") + hipGetErrorString(e));
-  return EQF_OK;
+  return eqf_dev_upload(s, field_ptr(s, field), field == EQF_DENSITY ? field_ptr(s, F_SC) : nullptr, amounts, 1);
 }
 
 extern "C" int eqf_set_frame_source(eqf_scene* s, int field, const float* amounts) {
@@ -539,15 +450,10 @@ extern "C" int eqf_set_frame_source(eqf_scene* s, int field, const float* amount
   float* dst = s->d_src + (int64_t)slot * s->fstride;
   if (!amounts) {
     HIPCHK(hipMemsetAsync(dst, 0, sizeof(float) * s->fstride, s->stream));
-  } else {
-    const int S = s->S;
-    s->hbuf.assign(s->fstride, 0.0f);
-    for (int y = 0; y < S; ++y)
-      for (int x = 0; x < S; ++x) s->hbuf[eqf_scene_off(s, x, y)] = amounts[(size_t)x + (size_t)y * S];
-    HIPCHK(hipMemcpyAsync(dst, s->hbuf.data(), sizeof(float) * s->fstride, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return EQF_OK;
   }
-  HIPCHK(hipStreamSynchronize(s->stream));
-  return EQF_OK;
+  return eqf_dev_upload(s, dst, nullptr, amounts, 0);
 }
 
 extern "C" int eqf_upload(eqf_scene* s, int field, const float* host) {

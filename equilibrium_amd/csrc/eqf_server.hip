@@ -63,7 +63,8 @@ struct eqf_server {
   uint64_t peak_bytes = 0;
   bool stop = false;
   std::thread th;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // ticks
+  hipStream_t astream = nullptr;  // admissions (overlap the running tick)
   SceneDesc* d_descs = nullptr;
   int desc_cap = 0;
   Clock::time_point t0;
@@ -89,7 +90,7 @@ static uint64_t estimate_bytes(const eqf_params& p) {
 
 static int admit(eqf_server* sv, Req* r, std::string& err) {
   eqf_scene* s = nullptr;
-  int rc = eqf_scene_create(sv->ctx, &r->p, &s);
+  int rc = eqf_internal_scene_create(sv->ctx, &r->p, sv->astream, &s);
   for (size_t k = 0; !rc && k + 4 <= r->rects.size(); k += 4)
     rc = eqf_fill_rect(s, r->rects[k], r->rects[k + 1], r->rects[k + 2], r->rects[k + 3]);
   if (!rc && !r->iu.empty()) rc = eqf_upload(s, EQF_VX, r->iu.data());
@@ -184,9 +185,16 @@ static void server_loop(eqf_server* sv) {
         launched = true;
       }
     }
-    // Admit queued requests while the tick runs.
+    // Admit queued requests while the tick runs: at least one per tick, then
+    // more only while the launch is still in flight (with nothing resident,
+    // for up to 50 ms), so admission never delays the resident scenes' ticks.
     std::vector<Req*> admitted;
+    const double t_adm0 = sv->now_ms();
     for (;;) {
+      if (!admitted.empty()) {
+        if (launched && hipStreamQuery(sv->stream) != hipErrorNotReady) break;
+        if (!launched && sv->now_ms() - t_adm0 > 50.0) break;
+      }
       Req* r = nullptr;
       {
         std::lock_guard<std::mutex> lk(sv->mu);
@@ -253,6 +261,9 @@ static void server_loop(eqf_server* sv) {
         r->t_done = sv->now_ms();
       }
       for (Req* r : admitted) {
+        // Staging buffers are only needed again for the final read-back; free
+        // them now that no tick is running (hipFree waits for the device).
+        eqf_dev_trim(r->scene);
         if (r->nframes <= 0) {  // nothing to run
           finish(sv, r);
           sv->used_bytes -= r->bytes;
@@ -288,7 +299,8 @@ extern "C" int eqf_server_create(eqf_ctx* ctx, const eqf_server_config* cfg, eqf
   sv->ctx = ctx;
   sv->cfg = *cfg;
   sv->t0 = Clock::now();
-  if (hipStreamCreate(&sv->stream) != hipSuccess) {
+  if (hipStreamCreate(&sv->stream) != hipSuccess || hipStreamCreate(&sv->astream) != hipSuccess) {
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
     delete sv;
     return eqf_internal_error(EQF_EDEVICE, "hipStreamCreate failed");
   }
@@ -432,6 +444,7 @@ extern "C" int eqf_server_destroy(eqf_server* sv) {
   (void)hipStreamSynchronize(sv->stream);
   (void)hipFree(sv->d_descs);
   (void)hipStreamDestroy(sv->stream);
+  (void)hipStreamDestroy(sv->astream);
   delete sv;
   return EQF_OK;
 }

@@ -1,0 +1,98 @@
+// The reference's own unit tests, restated against the C++ mirror of its API
+// (equilibrium_amd/host/fluid.hpp) over libeqfluid.so, plus a step-parity
+// check against the CPU oracle.  `--cpu` runs the tests that need no GPU.
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "../../equilibrium_amd/host/fluid.hpp"
+
+using namespace equilibrium;
+
+extern "C" {
+// oracle/eqf_oracle.c (test infrastructure, loaded only by this test)
+void ora_init_f32(float*, float*, float*, float*, float*, float*, uint8_t*, uint32_t, int);
+void ora_fill_rect(uint8_t*, uint32_t, int64_t, int64_t, int64_t, int64_t);
+void ora_run_f32(float*, float*, float*, float*, float*, float*, int64_t, float, float, float, int64_t, int64_t,
+                 const uint8_t*, int, const float*, const float*, const float*, int64_t);
+}
+
+static int failures = 0;
+#define CHECK(cond)                                                  \
+  do {                                                               \
+    if (!(cond)) {                                                   \
+      std::fprintf(stderr, "FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+      ++failures;                                                    \
+    }                                                                \
+  } while (0)
+
+// fluid.rs:626-635
+static void to_coordinate() {
+  const std::pair<int64_t, int64_t> coordinate{3, 4};
+  const int64_t size = 10;
+  const int64_t idx = coordinate.first + coordinate.second * size;  // idx!(3, 4, 10)
+  CHECK(Fluid::to_coordinate(idx, size) == coordinate);
+}
+
+// obstacle.rs:100-115
+static void rectangle_panics() {
+  bool thrown = false;
+  try {
+    Rectangle({50, 120}, {127, 110}, 128);
+  } catch (const std::invalid_argument& e) {
+    thrown = std::strcmp(e.what(), "Invalid input for Rectangle") == 0;
+  }
+  CHECK(thrown);
+  thrown = false;
+  try {
+    Rectangle({12, 12}, {10, 10}, 128);
+  } catch (const std::invalid_argument&) {
+    thrown = true;
+  }
+  CHECK(thrown);
+  Rectangle r = Rectangle::default_();
+  CHECK(r.get_approximate_points().size() == 2);
+}
+
+// renderer_helpers.rs:222-252
+static void mark_fluid_obstacles_default_current_simulation() {
+  CurrentSimulation sim;
+  sim.mark_fluid_obstacles();
+  int64_t walls = 0;
+  for (auto c : sim.fluid.cells_type()) walls += c == ContainerWall::DefaultWall;
+  const int64_t size = sim.fluid.simulation_configs.size;
+  const int64_t image_parameter = 2 * (size + (size - 2));
+  CHECK(walls == 30 * 30 + image_parameter);  // 1408
+}
+
+// Default simulation (size 128, dt 0.02, 16 frames = 16 sweeps per lin_solve,
+// default rectangle) run for 4 frames on the GPU == the CPU oracle, bit for bit.
+static void default_simulation_matches_oracle() {
+  CurrentSimulation sim;
+  sim.fluid.simulation_configs.frames = 4;
+  std::vector<std::vector<float>> got;
+  sim.simulate([&](int64_t, Fluid& f) { got = {f.density(), f.velocities_x(), f.velocities_y()}; });
+  const uint32_t S = 128;
+  std::vector<float> D(S * S), Sc(S * S), U(S * S), V(S * S), U0(S * S), V0(S * S);
+  std::vector<uint8_t> m(S * S);
+  ora_init_f32(D.data(), Sc.data(), U.data(), V.data(), U0.data(), V0.data(), m.data(), S, 2);
+  ora_fill_rect(m.data(), S, 80, 80, 110, 110);
+  ora_run_f32(D.data(), Sc.data(), U.data(), V.data(), U0.data(), V0.data(), S, 0.02f, 0.0f, 0.001f, 4, 4, m.data(), 0,
+              nullptr, nullptr, nullptr, 4);
+  CHECK(std::memcmp(got[0].data(), D.data(), D.size() * 4) == 0);
+  CHECK(std::memcmp(got[1].data(), U.data(), U.size() * 4) == 0);
+  CHECK(std::memcmp(got[2].data(), V.data(), V.size() * 4) == 0);
+}
+
+int main(int argc, char** argv) {
+  const bool gpu = !(argc > 1 && std::strcmp(argv[1], "--cpu") == 0);
+  to_coordinate();
+  rectangle_panics();
+  if (gpu) {
+    mark_fluid_obstacles_default_current_simulation();
+    default_simulation_matches_oracle();
+  }
+  std::printf("%s: %d failure(s)\n", gpu ? "cpu+gpu" : "cpu", failures);
+  return failures ? 1 : 0;
+}
