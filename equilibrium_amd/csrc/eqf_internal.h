@@ -16,7 +16,7 @@ struct eqf_ctx {
 struct eqf_scene {
   eqf_ctx* ctx = nullptr;
   eqf_params p{};
-  int S = 0, N = 0, ns = 0;
+  int S = 0, N = 0, ns = 0, P = 0;
   int layout = eqf::L_CYCLIC;
   int64_t fstride = 0;
   std::vector<uint8_t> mask;  // row-major, host copy

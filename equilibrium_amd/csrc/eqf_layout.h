@@ -5,15 +5,17 @@
 // HBM.  Two layouts exist:
 //
 //  * CYCLIC (exact wavefront mode).  Interior rows are grouped in strips of 64
-//    rows (strip s = rows 64s+1 .. 64s+64, lane r = row-in-strip).  Cell
-//    (c, j), 1 <= c, j <= N, with s = (j-1)/64, r = (j-1)%64, is stored at
-//    slot t = (c-1+r) mod N of strip s:  offset (s*N + t)*64 + r.  A wave that
-//    owns a strip walks the Gauss-Seidel anti-diagonal wavefront (lane r at
-//    column t-r+1 at step t) and every step touches one 256-byte slot row of
-//    each operand: fully coalesced, no LDS staging.  The ring follows the
-//    interior: top[c] (row 0), bottom[c] (row S-1), left[j] (column 0),
-//    right[j] (column S-1), 4*S floats; corners live in top[0], top[S-1],
-//    bottom[0], bottom[S-1].
+//    rows (strip s = rows 64s+1 .. 64s+64, lane r = row-in-strip).  With the
+//    period P = N rounded up to a multiple of 16, cell (c, j), 1 <= c, j <= N,
+//    s = (j-1)/64, r = (j-1)%64, is stored at slot t = (c-1+r) mod P of strip s:
+//    offset (s*P + t)*64 + r (the P-N slots per lane that hold no column are
+//    unused).  A wave that owns a strip walks the Gauss-Seidel anti-diagonal
+//    wavefront (lane r at column t-r+1 at step t) and every step touches one
+//    256-byte slot row of each operand: fully coalesced, no LDS staging; with
+//    P a multiple of 16 a block of 16 steps never wraps, so all its addresses
+//    are one base plus constants.  The ring follows the interior: top[c]
+//    (row 0), bottom[c] (row S-1), left[j] (column 0), right[j] (column S-1),
+//    4*S floats; corners live in top[0], top[S-1], bottom[0], bottom[S-1].
 //
 //  * ROWMAJOR (red-black mode).  Plain `x + y*S` like the reference
 //    (fluid.rs:31-35), ring included.
@@ -53,9 +55,10 @@ struct SceneDesc {
   const uint8_t* code;   // boundary codes, interior cells only (same indexing as fields)
   const uint8_t* rowf;   // [S] row j holds a NoWall cell
   const int2* bnd;       // (dst, src) field offsets: ROW list, COL list, PASSIVE list
-  float* hraw;           // cyclic: ns*N raw last-row values handed to the strip below
+  float* hraw;           // cyclic: 2*ns rows of P hand-off values between strips (see gs_wavefront)
   int64_t fstride;       // floats per field
   int S, N, ns;          // size, interior size, strips
+  int P;                 // cyclic period (N rounded up to a multiple of 16)
   int nbnd[3];           // list lengths per mode
   int diff_iters, press_iters;
   float a_visc, cr_visc; // diffuse coefficients for velocity (fluid.rs:286-294, 311)
@@ -67,24 +70,26 @@ struct SceneDesc {
   int pad_;
 };
 
+EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
+
 // Ring offset inside a cyclic field.
-EQF_HD inline int64_t cyc_ring(int N, int ns) { return (int64_t)ns * N * 64; }
+EQF_HD inline int64_t cyc_ring(int P, int ns) { return (int64_t)ns * P * 64; }
 
 // Offset of cell (x, y), 0 <= x, y <= S-1, inside one field.
-EQF_HD inline int64_t cyc_off(int x, int y, int S, int N, int ns) {
-  const int64_t ro = cyc_ring(N, ns);
+EQF_HD inline int64_t cyc_off(int x, int y, int S, int P, int ns) {
+  const int64_t ro = cyc_ring(P, ns);
   if (y == 0) return ro + x;
   if (y == S - 1) return ro + S + x;
   if (x == 0) return ro + 2 * S + y;
   if (x == S - 1) return ro + 3 * S + y;
   const int j = y - 1, s = j >> 6, r = j & 63;
   int t = x - 1 + r;
-  if (t >= N) t -= N;
-  return ((int64_t)s * N + t) * 64 + r;
+  if (t >= P) t -= P;
+  return ((int64_t)s * P + t) * 64 + r;
 }
 
-EQF_HD inline int64_t cyc_field_floats(int S, int N, int ns) {
-  int64_t n = cyc_ring(N, ns) + 4 * (int64_t)S;
+EQF_HD inline int64_t cyc_field_floats(int S, int P, int ns) {
+  int64_t n = cyc_ring(P, ns) + 4 * (int64_t)S;
   return (n + 63) & ~(int64_t)63;
 }
 

@@ -56,7 +56,7 @@ static int64_t resolved_press(const eqf_params& p) {
 }
 
 int64_t eqf_scene_off(const eqf_scene* s, int x, int y) {
-  return s->layout == L_CYCLIC ? cyc_off(x, y, s->S, s->N, s->ns) : (int64_t)x + (int64_t)y * s->S;
+  return s->layout == L_CYCLIC ? cyc_off(x, y, s->S, s->P, s->ns) : (int64_t)x + (int64_t)y * s->S;
 }
 
 static float* field_ptr(const eqf_scene* s, int f) { return s->d_fld + (int64_t)f * s->fstride; }
@@ -127,6 +127,7 @@ static void fill_desc(eqf_scene* s) {
   d.S = s->S;
   d.N = s->N;
   d.ns = s->ns;
+  d.P = s->P;
   for (int i = 0; i < 3; ++i) d.nbnd[i] = s->nbnd[i];
   d.diff_iters = (int)resolved_gs(p);
   d.press_iters = (int)resolved_press(p);
@@ -216,7 +217,8 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
   s->N = s->S - 2;
   s->ns = (s->N + 63) / 64;
   s->layout = p->gs_mode == EQF_GS_WAVEFRONT ? L_CYCLIC : L_ROWMAJOR;
-  s->fstride = s->layout == L_CYCLIC ? cyc_field_floats(s->S, s->N, s->ns) : rm_field_floats(s->S);
+  s->P = cyc_period(s->N);
+  s->fstride = s->layout == L_CYCLIC ? cyc_field_floats(s->S, s->P, s->ns) : rm_field_floats(s->S);
   hipError_t e = hipSuccess;
   if (stream) {
     s->stream = stream;
@@ -228,11 +230,11 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
     scene_free(s);
     return fail(EQF_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
-  const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->N * 64 : (size_t)s->S * s->S;
+  const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->P * 64 : (size_t)s->S * s->S;
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t b_fld = up(sizeof(float) * NFIELDS * (size_t)s->fstride);
   const size_t b_code = up(ncode), b_rowf = up((size_t)s->S);
-  const size_t b_hraw = up(sizeof(float) * ((size_t)s->ns * s->N + 4)), b_desc = up(sizeof(SceneDesc));
+  const size_t b_hraw = up(sizeof(float) * (2 * (size_t)s->ns * s->P + 4)), b_desc = up(sizeof(SceneDesc));
   e = hipMalloc(&s->d_main, b_fld + b_code + b_rowf + b_hraw + b_desc);
   if (e == hipSuccess) {
     char* base = (char*)s->d_main;

@@ -82,9 +82,9 @@ struct eqf_server {
 
 // Device bytes of a wavefront scene of this size with frame sources.
 static uint64_t estimate_bytes(const eqf_params& p) {
-  const int S = (int)p.size, N = S - 2, ns = (N + 63) / 64;
-  const uint64_t fs = (uint64_t)cyc_field_floats(S, N, ns);
-  return (NFIELDS + 3) * fs * 4 + (uint64_t)ns * N * 64 + (uint64_t)S + (uint64_t)N * 96 + (uint64_t)ns * N * 4 +
+  const int S = (int)p.size, N = S - 2, ns = (N + 63) / 64, P = cyc_period(N);
+  const uint64_t fs = (uint64_t)cyc_field_floats(S, P, ns);
+  return (NFIELDS + 3) * fs * 4 + (uint64_t)ns * P * 64 + (uint64_t)S + (uint64_t)N * 96 + 2ull * ns * P * 4 +
          sizeof(SceneDesc) + 4096;
 }
 

@@ -16,11 +16,11 @@ using namespace eqf;
 namespace {
 
 struct Geo {
-  int S, N, ns, layout;
+  int S, N, ns, P, layout;
 };
 
 __device__ __forceinline__ int64_t off_of(const Geo& g, int x, int y) {
-  return g.layout == L_CYCLIC ? cyc_off(x, y, g.S, g.N, g.ns) : (int64_t)x + (int64_t)y * g.S;
+  return g.layout == L_CYCLIC ? cyc_off(x, y, g.S, g.P, g.ns) : (int64_t)x + (int64_t)y * g.S;
 }
 
 // Fluid::new / Fluid::default state (fluid.rs:93-110, 527-570) for sizes
@@ -137,7 +137,7 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
   }
 }
 
-Geo geo_of(const eqf_scene* s) { return Geo{s->S, s->N, s->ns, s->layout}; }
+Geo geo_of(const eqf_scene* s) { return Geo{s->S, s->N, s->ns, s->P, s->layout}; }
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
@@ -230,7 +230,7 @@ int eqf_dev_rebuild_bnd(eqf_scene* s) {
   }
   e = hipMemsetAsync(d_cnt, 0, 3 * sizeof(int), s->stream);
   if (e == hipSuccess) {
-    const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->N * 64 : (size_t)n;
+    const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->P * 64 : (size_t)n;
     e = hipMemsetAsync(s->d_code, 0, ncode, s->stream);
   }
   if (e == hipSuccess) {

@@ -7,10 +7,11 @@
 //
 // Fields use the CYCLIC strip layout (eqf_layout.h): wave w owns 64-row
 // strips, lane r owns row 64s+r+1, and slot t of a strip holds column
-// (t-r) mod N + 1 of every row.  The reference's lexicographic Gauss-Seidel
-// (fluid.rs:312-324) is reproduced exactly by an anti-diagonal wavefront:
-// at strip-local step tau lane r updates column (tau-r) mod N + 1 of sweep
-// (tau-r) div N.  Its left/up neighbours are this sweep's values (own
+// (t-r) mod P + 1 of every row (P = N rounded up to a multiple of 16; slots
+// whose column would exceed N are padding).  The reference's lexicographic
+// Gauss-Seidel (fluid.rs:312-324) is reproduced exactly by an anti-diagonal
+// wavefront: at strip-local step tau lane r updates column (tau-r) mod P + 1
+// of sweep (tau-r) div P (and idles on padding slots).  Its left/up neighbours are this sweep's values (own
 // register / lane r-1 via DPP wave_shr), right/down neighbours are the
 // previous sweep's values after set_boundaries (slot t+1 of the lane / lane
 // r+1 via DPP wave_shl).  Every step reads and writes whole 256-byte slot
@@ -37,18 +38,24 @@ typedef GAS const float gcf;
 typedef GAS const uint8_t gcu8;
 typedef GAS const int gci;
 
+// Lane 0 (from_prev_lane) and lane 63 (from_next_lane) read 0 (bound_ctrl);
+// callers replace those lanes' values.
 __device__ __forceinline__ float from_prev_lane(float v) {  // lane i <- lane i-1 (DPP wave_shr:1)
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_next_lane(float v) {  // lane i <- lane i+1 (DPP wave_shl:1)
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 // Raw buffer descriptor (gfx9 family: DATA_FORMAT 32 in word 3).
+// Base and size are wave-uniform; readfirstlane keeps the descriptor in
+// scalar registers even where the compiler computed them in vector registers.
 __device__ __forceinline__ rsrc_t make_rsrc(GAS const void* base, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+  const uint64_t a = (uint64_t)(uintptr_t)(const void*)base;
+  const uint64_t au = (uint64_t)(uint32_t)uni((int)(uint32_t)a) | ((uint64_t)(uint32_t)uni((int)(a >> 32)) << 32);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)au, (short)0, uni((int)bytes), 0x00020000);
 }
 __device__ __forceinline__ float ldf(rsrc_t r, int voff, int soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
@@ -81,234 +88,11 @@ __device__ __forceinline__ void corners(gf* ring, int S, int N) {  // fluid.rs:2
   ring[2 * S - 1] = 0.5f * (ring[S + N] + ring[3 * S + N]);
 }
 
-// ------------------------------------------------------------------------
-// Gauss-Seidel wavefront (lin_solve, fluid.rs:301-325) with set_boundaries
-// fused.  MODE selects the set_boundaries rule.  D = prefetch depth (steps),
-// B = steps between workgroup barriers (a multiple of D).
-//
-// Ordering argument.  Lane r of strip s handles position u = tau - r, tau =
-// g - s*L.  A value stored at global step g is read by another wave through
-// a load issued at step >= g + B (a barrier lies in between; on gfx950 all
-// waves of a workgroup share one in-order vector-memory path, so a load
-// issued after the barrier observes stores issued before it).  With the lag
-// L = 63+B+D:
-//  * strip s+1 lane 0 reads strip s lane 63's raw value of the same column
-//    L-63-D >= B steps after it was stored;
-//  * strip s lane 63 reads strip s+1 lane 0's post-boundary value (and its
-//    AdjustColumn fix-up) of the previous sweep N+62-D-L-1 >= B steps after
-//    it was stored, which needs N >= 2(B+D)+2: true whenever a scene has more
-//    than one strip (N > 64);
-//  * within a wave, a cell's next-sweep load is issued N-2-D steps after its
-//    store: N >= D+3 (smaller grids take the sequential path).
-template <int MODE, int D, int B>
-__device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict__ X, gcf* __restrict__ X0,
-                                             float a, float cr, int iters) {
-  const int N = sc.N, S = sc.S, ns = sc.ns;
-  const int lane = threadIdx.x & 63;
-  const int wave = uni(threadIdx.x >> 6);
-  const int L = 63 + B + D;
-  const int NS64 = N * 64;
-  const int RO = ns * NS64;  // ring offset (floats)
-  gf* __restrict__ ring = X + RO;
-  const int G = (ns - 1) * L + 64 + iters * N + 1;
-  const int Gr = (G + B - 1) / B * B;
-
-  const bool has_strip = wave < ns;
-  const int s = has_strip ? wave : 0;
-  const int j = 64 * s + lane + 1;
-  const bool valid = has_strip && j <= N;
-  const int rl = min(63, N - 1 - 64 * s);  // last valid lane of the strip
-  const bool first = lane == 0;
-  const bool last = lane == rl;
-  const bool has_prev = s > 0;
-  const bool has_next = s < ns - 1;
-  const int jc = min(j, N);
-  const float ringL = ring[2 * S + jc];
-  const float ringR = ring[3 * S + jc];
-  const bool rowf = ((gcu8*)sc.rowf)[jc] != 0;
-
-  // Buffer descriptors over whole arrays; per-step addresses are a constant
-  // per-lane VGPR offset plus a uniform SGPR offset.  Lanes that must not
-  // store get an offset past the end of the buffer: the store is dropped.
-  const uint32_t fbytes = (uint32_t)sc.fstride * 4u;
-  const rsrc_t rX = make_rsrc(X, fbytes);
-  const rsrc_t rX0 = make_rsrc(X0, fbytes);
-  const rsrc_t rC = make_rsrc((gcu8*)sc.code, (uint32_t)RO);
-  const rsrc_t rH = make_rsrc((gcf*)sc.hraw, (uint32_t)(ns * N) * 4u);
-  const rsrc_t rU = has_prev ? rH : rX;
-  const int sb = s * NS64 * 4;                                // strip base (bytes)
-  const int ub = has_prev ? (s - 1) * N * 4 : (RO + 1) * 4;   // exu: hraw row above or top ring
-  const int dnb = has_next ? (s + 1) * NS64 * 4 : (RO + S + 1) * 4;  // exd: strip below or bottom ring
-  const int dns = has_next ? 8 : 2;                           // exd: slot stride 256 B or 4 B
-  const int vl = lane * 4;
-  const int OOB = (int)0x80000000;
-
-  const int lane_m1 = lane == 0 ? N - 1 : lane - 1;  // slot at which this lane is in column N
-  const unsigned itN = (unsigned)(iters * N);
-  const int lastN = (iters - 1) * N;
-
-  int tau = -s * L;                // uniform strip-local step
-  int tcur = ((tau % N) + N) % N;  // uniform slot of tau
-  int tld = (tcur + D) % N;        // uniform slot being prefetched
-
-  float fxs[D], fx0[D], fexu[D], fexd[D];
-  uint32_t fcode[D];
-
-#define EQF_PREFETCH(SL, T)                                                     \
-  {                                                                             \
-    const int t_ = (T);                                                         \
-    const int t1_ = t_ + 1 == N ? 0 : t_ + 1;                                   \
-    fxs[SL] = ldf(rX, vl, sb + t1_ * 256);                                      \
-    fx0[SL] = ldf(rX0, vl, sb + t_ * 256);                                      \
-    fcode[SL] = ldu8(rC, lane, s * NS64 + t_ * 64);                             \
-    fexu[SL] = ldf(rU, 0, ub + t_ * 4);                                         \
-    int tl_ = t_ - rl;                                                          \
-    tl_ += tl_ < 0 ? N : 0;                                                     \
-    fexd[SL] = ldf(rX, 0, dnb + (tl_ << dns));                                  \
-  }
-
-  if (has_strip) {
-    int t = tcur;
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      EQF_PREFETCH(i, t);
-      t = t + 1 == N ? 0 : t + 1;
-    }
-  }
-
-  float res_prev = 0.f, xl_prev = 0.f, xu_prev = 0.f, xself = 0.f, exd_prev = 0.f;
-  uint32_t code_prev = 0;
-  bool prev_active = false;
-  int tprev = 0;
-
-#pragma nounroll
-  for (int g0 = 0; g0 < Gr; g0 += B) {
-    if (has_strip) {
-#pragma unroll
-      for (int i = 0; i < B; ++i) {
-        const int sl = i % D;
-        const float xs = fxs[sl], x0v = fx0[sl], exu = fexu[sl], exd = fexd[sl];
-        const uint32_t code = fcode[sl];
-        const int u = tau - lane;
-        const bool active = (unsigned)u < itN;
-        const bool c1 = tcur == lane;
-        const bool cN = tcur == lane_m1;
-        int tl = tcur - rl;
-        tl += tl < 0 ? N : 0;  // slot index of the last lane's column - 1
-        float xl, xr, eu, ed;
-        if constexpr (MODE == M_PASSIVE) {
-          // set_boundaries(Passive) of the previous sweep copied the interior
-          // cell next to the ring into the ring (rows/columns that hold a
-          // NoWall cell).  That cell is the one updated now: its value from
-          // the previous sweep is xself.
-          const bool kp = u >= N;
-          const bool rc = kp && rowf;
-          const bool cc = kp && (code & C_COLF);
-          xl = c1 ? (rc ? xself : ringL) : res_prev;
-          xr = cN ? (rc ? xself : ringR) : xs;
-          eu = (!has_prev && cc) ? xself : exu;
-          ed = (!has_next && cc) ? xself : exd;
-        } else {
-          xl = c1 ? ringL : res_prev;
-          xr = cN ? ringR : xs;
-          eu = exu;
-          ed = exd;
-        }
-        const float dn = from_next_lane(xs);
-        const float up = from_prev_lane(res_prev);
-        const float xd = last ? ed : dn;
-        const float xu = first ? eu : up;
-        // fluid.rs:315-320
-        const float res = (x0v + a * (((xr + xl) + xd) + xu)) * cr;
-
-        if constexpr (MODE == M_PASSIVE) {
-          // Passive set_boundaries never rewrites interior cells: store now.
-          stf(res, rX, (active && valid) ? vl : OOB, sb + tcur * 256);
-          if (active && u >= lastN) {  // final sweep: materialise the ring
-            if (c1 && rowf && valid) ring[2 * S + j] = res;
-            if (cN && rowf && valid) ring[3 * S + j] = res;
-            if (first && s == 0 && (code & C_COLF)) ring[tcur + 1] = res;
-            if (last && s == ns - 1 && (code & C_COLF) && valid) ring[S + tl + 1] = res;
-          }
-        } else {
-          // Finalise the cell updated at the previous step (set_boundaries of
-          // this sweep, fluid.rs:151-178): its wall neighbours are known now.
-          float fin = res_prev;
-          const bool fl = (code_prev & C_FLUID) != 0;
-          if constexpr (MODE == M_ROW) {
-            const float vR = c1 ? ringR : res;  // right neighbour of the pending cell
-            fin = (fl && (code_prev & C_RWALL)) ? -vR : ((fl && (code_prev & C_LWALL)) ? -xl_prev : res_prev);
-          } else {
-            const float vD = from_next_lane(res);  // the lane below computed the pending column now
-            const float vDn = last ? exd_prev : vD;
-            const bool dok = !(last && has_next);  // otherwise the strip below writes it
-            fin = (fl && (code_prev & C_UWALL)) ? -xu_prev
-                                                 : ((fl && dok && (code_prev & C_DWALL)) ? -vDn : res_prev);
-          }
-          stf(fin, rX, (prev_active && valid) ? vl : OOB, sb + tprev * 256);
-          if constexpr (MODE == M_COL) {
-            // AdjustColumn value of the cell above (last row of the strip above)
-            int tu = tcur + 63;
-            tu -= tu >= N ? N : 0;
-            const bool fix = first && has_prev && active && (code & C_FIXUP);
-            stf(-res, rX, fix ? 0 : OOB, sb - NS64 * 4 + (tu * 64 + 63) * 4);
-          }
-          xl_prev = xl;
-          xu_prev = xu;
-          code_prev = code;
-          exd_prev = exd;
-          prev_active = active;
-          tprev = tcur;
-        }
-        // raw last-row value for the strip below
-        stf(res, rH, (last && has_next && active && valid) ? 0 : OOB, (s * N + tl) * 4);
-        res_prev = res;
-        xself = xs;
-        ++tau;
-        tcur = tcur + 1 == N ? 0 : tcur + 1;
-        EQF_PREFETCH(sl, tld);
-        tld = tld + 1 == N ? 0 : tld + 1;
-      }
-    }
-    __syncthreads();
-  }
-#undef EQF_PREFETCH
-
-  if (threadIdx.x == 0) corners(ring, S, N);
-  __syncthreads();
-}
-
-// Sequential lin_solve for tiny grids (N < D+3): one thread runs the
-// reference loops on the cyclic layout, set_boundaries through the lists.
-__device__ __forceinline__ void gs_serial(const SceneDesc& sc, int mode, gf* X, gcf* X0, float a, float cr,
-                                          int iters) {
-  const int S = sc.S, N = sc.N, ns = sc.ns;
-  gci* list = (gci*)sc.bnd + 2 * (mode == M_ROW ? 0 : mode == M_COL ? sc.nbnd[0] : sc.nbnd[0] + sc.nbnd[1]);
-  const int nl = sc.nbnd[mode];
-  const float sign = mode == M_PASSIVE ? 1.0f : -1.0f;
-  gf* ring = X + cyc_ring(N, ns);
-  if (threadIdx.x == 0) {
-    for (int kk = 0; kk < iters; ++kk) {
-      for (int y = 1; y <= N; ++y)
-        for (int x = 1; x <= N; ++x) {
-          X[cyc_off(x, y, S, N, ns)] =
-              (X0[cyc_off(x, y, S, N, ns)] +
-               a * (((X[cyc_off(x + 1, y, S, N, ns)] + X[cyc_off(x - 1, y, S, N, ns)]) + X[cyc_off(x, y + 1, S, N, ns)]) +
-                    X[cyc_off(x, y - 1, S, N, ns)])) *
-              cr;
-        }
-      for (int e = 0; e < nl; ++e) X[list[2 * e]] = sign * X[list[2 * e + 1]];
-      corners(ring, S, N);
-    }
-  }
-  __syncthreads();
-}
-
 // set_boundaries as a list pass: x[dst] = -x[src] (ROW/COL) or x[dst] = x[src]
 // (PASSIVE), then the four corners.  Up to two fields share one pass.
 __device__ __forceinline__ void bnd_lists(const SceneDesc& sc, int mode0, gf* x0, int mode1, gf* x1) {
   const int S = sc.S, N = sc.N;
-  const int ro = N * 64 * sc.ns;
+  const int ro = sc.P * 64 * sc.ns;
   const int off0 = mode0 == M_ROW ? 0 : mode0 == M_COL ? sc.nbnd[0] : sc.nbnd[0] + sc.nbnd[1];
   const int off1 = mode1 == M_ROW ? 0 : mode1 == M_COL ? sc.nbnd[0] : sc.nbnd[0] + sc.nbnd[1];
   const int n0 = sc.nbnd[mode0], n1 = x1 ? sc.nbnd[mode1] : 0;
@@ -327,53 +111,245 @@ __device__ __forceinline__ void bnd_lists(const SceneDesc& sc, int mode0, gf* x0
   __syncthreads();
 }
 
+// Bits i of [lo, hi) within a block of B <= 16 steps.
+__device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
+  lo = min(max(lo, 0), B);
+  hi = min(max(hi, 0), B);
+  return ((1u << hi) - 1u) & ~((1u << lo) - 1u);
+}
+
+// ------------------------------------------------------------------------
+// Gauss-Seidel wavefront (lin_solve, fluid.rs:301-325) with set_boundaries
+// fused.  MODE selects the set_boundaries rule.  D = prefetch depth (steps),
+// B = steps between workgroup barriers.
+//
+// Lane r of strip s works at position u = tau - r of the strip-local step
+// tau = g - s*L: column (u mod P) + 1 of sweep u div P (idle when that column
+// is past N: the P-N padding slots).  P and L are multiples of 16 and B
+// divides 16, so a block of B steps never wraps: every stream of a block is
+// one uniform slot base (soffset) plus a compile-time immediate, and the
+// per-lane conditions of the block (active steps, first/last column) are
+// bit masks and step indices computed once per block.
+//
+// Strips hand values to each other through H (2*ns rows of P floats, indexed
+// by the READER's slot so reads need no wrap):
+//   row 0            top ring values for strip 0's lane 0 (copied at entry)
+//   row 1+s          raw last-row values of strip s, read by lane 0 of s+1
+//   row ns-1+s       post-boundary row-0 values of strip s, read by the last
+//                    lane of strip s-1 in the next sweep
+//   row 2ns-1        bottom ring values for the last strip's last lane
+//
+// Ordering argument.  A value stored at global step g is read by another
+// wave through a load issued at a step >= g + B (a barrier lies in between;
+// on gfx950 the waves of a workgroup share one in-order vector-memory path,
+// so a load issued after the barrier observes stores issued before it).
+// With L >= 64+B+D: strip s+1 lane 0 reads strip s's raw last-row value of a
+// column L-63-D >= B steps after it was stored; strip s's last lane reads
+// strip s+1's post-boundary row-0 value (and the AdjustColumn fix-up) of the
+// previous sweep P+61-D-L >= B steps after it was stored, which holds for
+// P >= 80 (any scene with more than one strip).  Within a wave a cell's next
+// sweep load is issued P-2-D >= 0 steps after its store.
+//
+// PASSIVE: set_boundaries(Passive) copies the interior cell next to the ring
+// into the ring (rows/columns holding a NoWall cell, fluid.rs:179-187).  The
+// sweep reads such a ring value as `xself`, the previous sweep's value of the
+// cell being updated, and the ring itself is written once after the last
+// sweep by the list pass.
+template <int MODE, int D, int B>
+__device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict__ X, gcf* __restrict__ X0,
+                                             float a, float cr, int iters) {
+  static_assert(16 % B == 0 && D < B, "B must divide 16 and exceed the prefetch depth");
+  constexpr int L = (64 + B + D + B - 1) / B * B;  // strip lag, a multiple of B
+  const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
+  const int lane = threadIdx.x & 63;
+  const int wave = uni(threadIdx.x >> 6);
+  const int RO = ns * P * 64;
+  gf* __restrict__ ring = X + RO;
+  gf* __restrict__ H = (gf*)sc.hraw;
+  const int G = (ns - 1) * L + 64 + iters * P + 1;
+  const int Gr = (G + B - 1) / B * B;
+
+  const bool has_strip = wave < ns;
+  const int s = has_strip ? wave : 0;
+  const int j = 64 * s + lane + 1;
+  const bool valid = has_strip && j <= N;
+  const int rl = min(63, N - 1 - 64 * s);  // last valid lane of the strip
+  const bool first = lane == 0;
+  const bool last = lane == rl;
+  const bool has_prev = s > 0;
+  const bool has_next = s < ns - 1;
+  const int jc = min(j, N);
+  const float ringL = ring[2 * S + jc];
+  const float ringR = ring[3 * S + jc];
+  const bool rowf = ((gcu8*)sc.rowf)[jc] != 0;
+
+  // Values from outside the strip for the edge lanes.  Lane 0 reads its up
+  // neighbour from an H row (top ring or the strip above).  The last valid
+  // lane reads its down neighbour from an H row when it is lane 63 (the
+  // strip below's post row, seeded here with its entry row 0 for sweep 0, or
+  // the bottom ring); otherwise (last strip, N not a multiple of 64) the
+  // bottom ring is mirrored into the unused row below it, lane rl+1 of the
+  // strip, and arrives through the ordinary lane shift.  Entry ring values
+  // are what Passive sweeps need: they read the changing ring through xself.
+  if (has_strip && s == 0)
+    for (int t = lane; t < P; t += 64) H[t] = t < N ? ring[t + 1] : 0.0f;
+  if (has_strip && s > 0)  // entry row 0 of this strip: the "previous sweep" of sweep 0
+    for (int t = lane; t < P; t += 64) {
+      int c = t - 63;  // column - 1 of the reading lane 63 of the strip above at its slot t
+      c += c < 0 ? P : 0;
+      H[(ns - 1 + s) * P + t] = c < N ? X[(s * P + c) * 64] : 0.0f;
+    }
+  if (has_strip && s == ns - 1)
+    for (int t = lane; t < P; t += 64) {
+      int c = t - rl;
+      c += c < 0 ? P : 0;
+      if (rl == 63) {
+        H[(2 * ns - 1) * P + t] = c < N ? ring[S + c + 1] : 0.0f;
+      } else {  // row N+1 at lane rl+1: slot (c - 1 + rl + 1) mod P holds column c
+        int cm = c - 1;
+        cm += cm < 0 ? P : 0;
+        X[(s * P + t) * 64 + rl + 1] = cm < N ? ring[S + cm + 1] : 0.0f;
+      }
+    }
+  __syncthreads();
+
+  const uint32_t fbytes = (uint32_t)sc.fstride * 4u;
+  const rsrc_t rX = make_rsrc(X, fbytes);
+  const rsrc_t rX0 = make_rsrc(X0, fbytes);
+  const rsrc_t rC = make_rsrc((gcu8*)sc.code, (uint32_t)RO);
+  const rsrc_t rH = make_rsrc(H, (uint32_t)(2 * ns * P) * 4u);
+  const int OOB = (int)0x80000000;  // voffset that drops a store / reads 0
+  const int vl = lane * 4;
+  const int vls = valid ? vl : OOB;                   // stores: rows past N stay untouched
+  const bool last_e = last && lane == 63;             // down neighbour from H
+  const bool top = first && !has_prev;                // up neighbour is the top ring
+  const bool bot = last && !has_next;                 // down neighbour is the bottom ring
+  const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
+  const int dn_row = has_next ? ns + s : 2 * ns - 1;  // post row of the strip below, or the bottom ring
+  const int ve = first ? up_row * P * 4 : (last_e ? dn_row * P * 4 : OOB);
+  const int hraw_base = uni((1 + s) * P * 4);              // this strip's raw last row (if has_next)
+  const int hpost_base = uni((ns - 1 + s) * P * 4);        // this strip's post row 0 (if has_prev)
+  const int vhraw = (last && valid && has_next) ? 0 : OOB;
+  const int vhpost = (first && has_prev) ? 0 : OOB;
+  const int sb = uni(s * P * 256);                        // strip base in X (bytes)
+  const int cb = uni(s * P * 64);                        // strip base in the code array (bytes)
+  const int fxb = uni(has_prev ? sb - P * 256 + 252 : 0);  // lane 63 of the strip above (bytes)
+  const int itP = uni(iters * P);
+  const int o64 = uni(64 % P);
+
+  float fxs[D], fx0[D], fe[D];
+  uint32_t fcode[D];
+
+  // Strip s idles for its first s*L/B blocks (every lane before its first
+  // sweep): those blocks are barriers only.  Its first working block then
+  // starts at strip-local step tau = 0, slot 0.
+  const int nblk = Gr / B;
+  const int head = has_strip ? s * (L / B) : nblk;
+  int tau = 0;       // strip-local step of the block's first step
+  int tb = 0;        // its slot (uniform)
+  int tbp = P - B;   // previous block's slot
+
+  float res_prev = 0.f, xl_prev = 0.f, xu_prev = 0.f, xd_prev = 0.f, xself = 0.f;
+  uint32_t code_prev = 0;
+  bool prev_active = false, cN_prev = false;
+
+  // One block of B steps: EDGE blocks predicate every store on the lane's
+  // step lying in the sweep range; interior blocks (every lane's steps and
+  // the pending step before them inside the range) only meet padding slots
+  // (columns past N, never read as cell values) and store unconditionally.
+  int blk = 0;
+#pragma nounroll
+  for (; blk < head; ++blk) __syncthreads();
+  if (has_strip) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) {  // prefetch the first D steps (slot i)
+      fxs[i] = ldf(rX, vl + (i + 1) * 256, sb);
+      fx0[i] = ldf(rX0, vl + i * 256, sb);
+      fcode[i] = ldu8(rC, lane + i * 64, cb);
+      fe[i] = ldf(rH, ve + i * 4, 0);
+    }
+#pragma nounroll
+    while (blk < nblk) {
+      if (tau >= 64 && tau + B <= itP) {
+#pragma nounroll
+        do {
+          {
+            constexpr bool EDGE = false;
+#include "eqf_gs_block.inc"
+          }
+          ++blk;
+        } while (tau + B <= itP);
+      } else if (tau < itP + 64) {
+        {
+          constexpr bool EDGE = true;
+#include "eqf_gs_block.inc"
+        }
+        ++blk;
+      } else {
+        __syncthreads();
+        ++blk;
+      }
+    }
+  }
+
+  if constexpr (MODE == M_PASSIVE) {
+    // set_boundaries(Passive) of the last sweep (its corners included)
+    bnd_lists(sc, M_PASSIVE, X, M_PASSIVE, nullptr);
+  } else {
+    if (threadIdx.x == 0) corners(ring, S, N);
+    __syncthreads();
+  }
+}
+
 // project, first half (fluid.rs:339-349): div and p = 0 on interior cells,
 // then set_boundaries(Passive) on both (fluid.rs:351-352).
 __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__ u, gcf* __restrict__ v,
                                           gf* __restrict__ div, gf* __restrict__ p) {
-  const int N = sc.N, S = sc.S, ns = sc.ns;
+  const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
-  const int NS64 = N * 64;
-  gcf* uring = u + ns * NS64;
-  gcf* vring = v + ns * NS64;
+  const int PS = P * 64;
+  gcf* uring = u + ns * PS;
+  gcf* vring = v + ns * PS;
   const float sf = sc.sf;
   for (int s = uni(threadIdx.x >> 6); s < ns; s += (blockDim.x >> 6)) {
     const int j = 64 * s + lane + 1;
-    const bool valid = j <= N;
+    const bool row_ok = j <= N;
     const int jc = min(j, N);
     const int rl = min(63, N - 1 - 64 * s);
     const bool first = lane == 0, last = lane == rl;
     const bool has_prev = s > 0, has_next = s < ns - 1;
-    gcf* us = u + s * NS64;
-    gcf* vs = v + s * NS64;
-    gcf* vnext = v + (has_next ? s + 1 : s) * NS64;
-    gcf* vprev = v + (has_prev ? s - 1 : s) * NS64;
-    gf* ds = div + s * NS64;
-    gf* ps = p + s * NS64;
+    gcf* us = u + s * PS;
+    gcf* vs = v + s * PS;
+    gcf* vnext = v + (has_next ? s + 1 : s) * PS;
+    gcf* vprev = v + (has_prev ? s - 1 : s) * PS;
+    gf* ds = div + s * PS;
+    gf* ps = p + s * PS;
     const float uL = uring[2 * S + jc], uR = uring[3 * S + jc];
-    const int lane_m1 = lane == 0 ? N - 1 : lane - 1;
-    float up_ = us[(N - 1) * 64 + lane], uc = us[lane];
-    float vp_ = vs[(N - 1) * 64 + lane], vc = vs[lane];
-    for (int t = 0; t < N; ++t) {
-      const int t1 = t + 1 == N ? 0 : t + 1;
+    const int lane_nm1 = (lane + N - 1) % P;
+    float up_ = us[(P - 1) * 64 + lane], uc = us[lane];
+    float vp_ = vs[(P - 1) * 64 + lane], vc = vs[lane];
+    for (int t = 0; t < P; ++t) {
+      const int t1 = t + 1 == P ? 0 : t + 1;
       const float un = us[t1 * 64 + lane];
       const float vn = vs[t1 * 64 + lane];
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;  // column - 1 of this lane at slot t
       int tl = t - rl;
-      tl += tl < 0 ? N : 0;  // column of the last lane is tl+1
+      tl += tl < 0 ? P : 0;  // column - 1 of the last lane
       int tu = t + 63;
-      tu -= tu >= N ? N : 0;  // strip above, lane 63, column t+1
-      gcf* pdn = has_next ? vnext + tl * 64 : vring + S + tl + 1;
-      gcf* pup = has_prev ? vprev + tu * 64 + 63 : vring + t + 1;
+      tu -= tu >= P ? P : 0;  // strip above, lane 63, column t+1
+      gcf* pdn = has_next ? vnext + tl * 64 : vring + S + min(tl, N - 1) + 1;
+      gcf* pup = has_prev ? vprev + tu * 64 + 63 : vring + min(t, N - 1) + 1;
       const float vdn_e = *pdn, vup_e = *pup;
       // neighbours (fluid.rs:342-344)
-      const float uplus = t == lane_m1 ? uR : un;
+      const float uplus = t == lane_nm1 ? uR : un;
       const float uminus = t == lane ? uL : up_;
       const float vdn_l = from_next_lane(vn);
       const float vup_l = from_prev_lane(vp_);
       const float vplus = last ? vdn_e : vdn_l;
       const float vminus = first ? vup_e : vup_l;
       const float d = -0.5f * (uplus - uminus + vplus - vminus) / sf;
-      if (valid) {
+      if (row_ok && c0 < N) {
         ds[t * 64 + lane] = d;
         ps[t * 64 + lane] = 0.0f;
       }
@@ -391,43 +367,45 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
 // then set_boundaries(AdjustRow, u) and (AdjustColumn, v).
 __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__ u, gf* __restrict__ v,
                                            gcf* __restrict__ p) {
-  const int N = sc.N, S = sc.S, ns = sc.ns;
+  const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
-  const int NS64 = N * 64;
-  gcf* pring = p + ns * NS64;
+  const int PS = P * 64;
+  gcf* pring = p + ns * PS;
   const float sf = sc.sf;
   for (int s = uni(threadIdx.x >> 6); s < ns; s += (blockDim.x >> 6)) {
     const int j = 64 * s + lane + 1;
-    const bool valid = j <= N;
+    const bool row_ok = j <= N;
     const int jc = min(j, N);
     const int rl = min(63, N - 1 - 64 * s);
     const bool first = lane == 0, last = lane == rl;
     const bool has_prev = s > 0, has_next = s < ns - 1;
-    gcf* ps = p + s * NS64;
-    gcf* pnext = p + (has_next ? s + 1 : s) * NS64;
-    gcf* pprev = p + (has_prev ? s - 1 : s) * NS64;
-    gf* us = u + s * NS64;
-    gf* vs = v + s * NS64;
+    gcf* ps = p + s * PS;
+    gcf* pnext = p + (has_next ? s + 1 : s) * PS;
+    gcf* pprev = p + (has_prev ? s - 1 : s) * PS;
+    gf* us = u + s * PS;
+    gf* vs = v + s * PS;
     const float pL = pring[2 * S + jc], pR = pring[3 * S + jc];
-    const int lane_m1 = lane == 0 ? N - 1 : lane - 1;
-    float pp = ps[(N - 1) * 64 + lane], pc = ps[lane];
-    for (int t = 0; t < N; ++t) {
-      const int t1 = t + 1 == N ? 0 : t + 1;
+    const int lane_nm1 = (lane + N - 1) % P;
+    float pp = ps[(P - 1) * 64 + lane], pc = ps[lane];
+    for (int t = 0; t < P; ++t) {
+      const int t1 = t + 1 == P ? 0 : t + 1;
       const float pn = ps[t1 * 64 + lane];
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;
       int tl = t - rl;
-      tl += tl < 0 ? N : 0;
+      tl += tl < 0 ? P : 0;
       int tu = t + 63;
-      tu -= tu >= N ? N : 0;
-      gcf* qdn = has_next ? pnext + tl * 64 : pring + S + tl + 1;
-      gcf* qup = has_prev ? pprev + tu * 64 + 63 : pring + t + 1;
+      tu -= tu >= P ? P : 0;
+      gcf* qdn = has_next ? pnext + tl * 64 : pring + S + min(tl, N - 1) + 1;
+      gcf* qup = has_prev ? pprev + tu * 64 + 63 : pring + min(t, N - 1) + 1;
       const float pdn_e = *qdn, pup_e = *qup;
-      const float pplus = t == lane_m1 ? pR : pn;
+      const float pplus = t == lane_nm1 ? pR : pn;
       const float pminus = t == lane ? pL : pp;
       const float pdn_l = from_next_lane(pn);
       const float pup_l = from_prev_lane(pp);
       const float pdn = last ? pdn_e : pdn_l;
       const float pup = first ? pup_e : pup_l;
-      if (valid) {
+      if (row_ok && c0 < N) {
         const int o = t * 64 + lane;
         us[o] = us[o] - 0.5f * (pplus - pminus) * sf;
         vs[o] = vs[o] - 0.5f * (pdn - pup) * sf;
@@ -466,11 +444,11 @@ __device__ __forceinline__ Trace back_trace(int c, int j, float uu, float vv, fl
   r.j1 = sat_idx(j1, S);
   return r;
 }
-__device__ __forceinline__ float gather(gcf* d0, const Trace& r, int S, int N, int ns) {
-  const float a = d0[cyc_off(r.i0, r.j0, S, N, ns)];
-  const float b = d0[cyc_off(r.i0, r.j1, S, N, ns)];
-  const float cc = d0[cyc_off(r.i1, r.j0, S, N, ns)];
-  const float d = d0[cyc_off(r.i1, r.j1, S, N, ns)];
+__device__ __forceinline__ float gather(gcf* d0, const Trace& r, int S, int P, int ns) {
+  const float a = d0[cyc_off(r.i0, r.j0, S, P, ns)];
+  const float b = d0[cyc_off(r.i0, r.j1, S, P, ns)];
+  const float cc = d0[cyc_off(r.i1, r.j0, S, P, ns)];
+  const float d = d0[cyc_off(r.i1, r.j1, S, P, ns)];
   return r.s0 * (r.t0 * a + r.t1 * b) + r.s1 * (r.t0 * cc + r.t1 * d);
 }
 
@@ -483,53 +461,52 @@ __device__ __forceinline__ float gather(gcf* d0, const Trace& r, int S, int N, i
 __device__ __forceinline__ void advect_phase(const SceneDesc& sc, int mode0, gf* __restrict__ d0out,
                                              gcf* __restrict__ d0in, int mode1, gf* __restrict__ d1out,
                                              gcf* __restrict__ d1in, gcf* __restrict__ u, gcf* __restrict__ v) {
-  const int N = sc.N, S = sc.S, ns = sc.ns;
+  const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
-  const int NS64 = N * 64;
+  const int PS = P * 64;
   const float dt0 = sc.dt0, sf = sc.sf, sfm1 = sc.sfm1;
   const bool two = d1out != nullptr;
   for (int s = uni(threadIdx.x >> 6); s < ns; s += (blockDim.x >> 6)) {
     const int j = 64 * s + lane + 1;
-    const bool valid = j <= N;
+    const bool row_ok = j <= N;
     const int jc = min(j, N);
-    gcf* us = u + s * NS64;
-    gcf* vs = v + s * NS64;
+    gcf* us = u + s * PS;
+    gcf* vs = v + s * PS;
     // pass A: first trigger column of this row
     int trig = N + 1;
-    for (int t = 0; t < N; ++t) {
-      int c = t - lane;
-      c += c < 0 ? N : 0;
-      c += 1;
-      const Trace r = back_trace(c, jc, us[t * 64 + lane], vs[t * 64 + lane], dt0, sf, sfm1, S);
-      if (r.trig && c < trig) trig = c;
+    for (int t = 0; t < P; ++t) {
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;
+      if (c0 < N) {
+        const Trace r = back_trace(c0 + 1, jc, us[t * 64 + lane], vs[t * 64 + lane], dt0, sf, sfm1, S);
+        if (r.trig && c0 + 1 < trig) trig = c0 + 1;
+      }
     }
     // pass B
-    const int ro = ns * NS64;
+    const int ro = ns * PS;
     const float left0 = d0out[ro + 2 * S + jc];
     const float left1 = two ? d1out[ro + 2 * S + jc] : 0.0f;
-    gf* o0 = d0out + s * NS64;
-    gf* o1 = two ? d1out + s * NS64 : d0out;
-    int tp = N - 1;
-    for (int t = 0; t < N; ++t) {
-      int c = t - lane;
-      c += c < 0 ? N : 0;
-      c += 1;
-      if (valid && c <= trig) {
+    gf* o0 = d0out + s * PS;
+    gf* o1 = two ? d1out + s * PS : d0out;
+    for (int t = 0; t < P; ++t) {
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;
+      const int c = c0 + 1;
+      if (row_ok && c <= N && c <= trig) {
         const int o = t * 64 + lane;
-        const bool copy_left = c == trig && c == 1;
-        if (copy_left) {  // trigger in column 1: copy the ring cell (fluid.rs:421)
+        if (c == trig && c == 1) {  // trigger in column 1: copy the ring cell (fluid.rs:421)
           o0[o] = left0;
           if (two) o1[o] = left1;
         } else {
           // c < trig: this cell; c == trig > 1: the left neighbour's value
+          const int tp = t == 0 ? P - 1 : t - 1;
           const int tt = c == trig ? tp : t;
           const int cc = c == trig ? c - 1 : c;
           const Trace r = back_trace(cc, j, us[tt * 64 + lane], vs[tt * 64 + lane], dt0, sf, sfm1, S);
-          o0[o] = gather(d0in, r, S, N, ns);
-          if (two) o1[o] = gather(d1in, r, S, N, ns);
+          o0[o] = gather(d0in, r, S, P, ns);
+          if (two) o1[o] = gather(d1in, r, S, P, ns);
         }
       }
-      tp = t;
     }
   }
   __syncthreads();
@@ -576,9 +553,6 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc) {
   gf* V = f + F_V * n;
   gf* U0 = f + F_U0 * n;
   gf* V0 = f + F_V0 * n;
-  // The wavefront needs N >= D+3 (a cell's next-sweep load must be issued
-  // after its store); more than one strip implies N > 64 >= 2*(B+D)+2.
-  const bool serial = sc.N < D + 3;
   if (sc.has_src) add_sources_phase(sc, f);
 #pragma nounroll
   for (int ph = 0; ph < 12; ++ph) {
@@ -601,8 +575,7 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc) {
     }
     if (kind == PK_GS) {
       if (iters > 0) {
-        if (serial) gs_serial(sc, mode, x, y, a, cr, iters);
-        else if (mode == M_ROW) gs_wavefront<M_ROW, D, B>(sc, x, y, a, cr, iters);
+        if (mode == M_ROW) gs_wavefront<M_ROW, D, B>(sc, x, y, a, cr, iters);
         else if (mode == M_COL) gs_wavefront<M_COL, D, B>(sc, x, y, a, cr, iters);
         else gs_wavefront<M_PASSIVE, D, B>(sc, x, y, a, cr, iters);
       }
