@@ -70,6 +70,16 @@ struct SceneDesc {
   int pad_;
 };
 
+// Exact-mode wavefront schedule: prefetch depth (steps) and steps between
+// workgroup barriers.  Compile-time so that every access offset is an
+// immediate.
+#ifndef EQF_GS_D
+#define EQF_GS_D 8
+#endif
+#ifndef EQF_GS_B
+#define EQF_GS_B 16
+#endif
+
 EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
 
 // Ring offset inside a cyclic field.

@@ -490,7 +490,7 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
   const int W = waves_for(s->ns);
   while (nframes > 0) {
     const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
-    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 1024>), dim3(1), dim3(64 * W), 0, s->stream, s->d_desc, nf);
+    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(1), dim3(64 * W), 0, s->stream, s->d_desc, nf);
     HIPCHK(hipGetLastError());
     nframes -= nf;
   }
@@ -574,9 +574,9 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
 hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int maxns, int nframes, hipStream_t st) {
   const int W = waves_for(maxns);
   if (W <= 8)
-    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 512>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
+    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
   else
-    hipLaunchKernelGGL((eqf_frames_cyclic<8, 16, 1024>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
+    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
   return hipGetLastError();
 }
 

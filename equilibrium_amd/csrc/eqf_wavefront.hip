@@ -37,14 +37,26 @@ typedef GAS float gf;
 typedef GAS const float gcf;
 typedef GAS const uint8_t gcu8;
 typedef GAS const int gci;
+typedef float f4 __attribute__((ext_vector_type(4)));  // elementwise IEEE adds, no contraction
+typedef GAS f4 gf4;
+typedef GAS const f4 gcf4;
 
-// Lane 0 (from_prev_lane) and lane 63 (from_next_lane) read 0 (bound_ctrl);
-// callers replace those lanes' values.
-__device__ __forceinline__ float from_prev_lane(float v) {  // lane i <- lane i-1 (DPP wave_shr:1)
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+// Lane shifts through DPP.  The lane without a source (lane 0 for wave_shr,
+// lane 63 for wave_shl) keeps `edge` (bound_ctrl off).  A DPP move reads its
+// source lane only if that lane is enabled in EXEC, so the move must run
+// with the whole wave enabled: the empty asm pins it where it is written, out
+// of any branch the compiler would otherwise build from a select around it.
+__device__ __forceinline__ float from_prev_lane(float v, float edge = 0.0f) {  // lane i <- lane i-1 (wave_shr:1)
+  float r = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                  __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+  asm volatile("" : "+v"(r));
+  return r;
 }
-__device__ __forceinline__ float from_next_lane(float v) {  // lane i <- lane i+1 (DPP wave_shl:1)
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+__device__ __forceinline__ float from_next_lane(float v, float edge = 0.0f) {  // lane i <- lane i+1 (wave_shl:1)
+  float r = __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                                  __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+  asm volatile("" : "+v"(r));
+  return r;
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -222,6 +234,8 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict
   const int vl = lane * 4;
   const int vls = valid ? vl : OOB;                   // stores: rows past N stay untouched
   const bool last_e = last && lane == 63;             // down neighbour from H
+  const bool mir = !has_next && rl < 63 && lane == rl + 1;  // holds the mirrored bottom ring
+  const int vmir = (last && !has_next && rl < 63) ? (rl + 1) * 4 : OOB;  // Passive: refresh the mirror
   const bool top = first && !has_prev;                // up neighbour is the top ring
   const bool bot = last && !has_next;                 // down neighbour is the bottom ring
   const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
@@ -301,6 +315,26 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict
   }
 }
 
+// The one-pass phases below walk each strip in blocks of 16 slots (P is a
+// multiple of 16, so a block never wraps): all loads of a block are issued
+// before any of its values is used, so a wave keeps ~50 loads in flight
+// instead of waiting out one memory latency per slot.
+
+// Offset, from the lane's edge base, of the value beyond the strip edge at
+// slot t: lane 0 reads lane 63 of the strip above at the same column (slot
+// t+63) or the top ring; the other lanes read lane 0 of the strip below at
+// the last lane's column (slot t-rl) or the bottom ring.  Lane 0 uses its
+// value as its up neighbour; lane 63 (a full strip) or lane rl+1 (the last,
+// partial strip) hands its value to the last lane rl through the lane shift.
+__device__ __forceinline__ int edge_index(int t, int lane, int rl, bool has_prev, bool has_next, int N, int P,
+                                          int S) {
+  int tl = t - rl;
+  tl += tl < 0 ? P : 0;
+  int tu = t + 63;
+  tu -= tu >= P ? P : 0;
+  return lane == 0 ? (has_prev ? tu * 64 : min(t, N - 1) + 1) : (has_next ? tl * 64 : S + min(tl, N - 1) + 1);
+}
+
 // project, first half (fluid.rs:339-349): div and p = 0 on interior cells,
 // then set_boundaries(Passive) on both (fluid.rs:351-352).
 __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__ u, gcf* __restrict__ v,
@@ -316,55 +350,57 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
     const bool row_ok = j <= N;
     const int jc = min(j, N);
     const int rl = min(63, N - 1 - 64 * s);
-    const bool first = lane == 0, last = lane == rl;
     const bool has_prev = s > 0, has_next = s < ns - 1;
-    gcf* us = u + s * PS;
-    gcf* vs = v + s * PS;
-    gcf* vnext = v + (has_next ? s + 1 : s) * PS;
-    gcf* vprev = v + (has_prev ? s - 1 : s) * PS;
-    gf* ds = div + s * PS;
-    gf* ps = p + s * PS;
+    gcf* us = u + s * PS + lane;
+    gcf* vs = v + s * PS + lane;
+    // lane 0 reads the strip above (its lane 63) or the top ring, every other
+    // lane the strip below (its lane 0) or the bottom ring
+    gcf* eb = lane == 0 ? (has_prev ? v + (s - 1) * PS + 63 : vring) : (has_next ? v + (s + 1) * PS : vring);
+    gf* ds = div + s * PS + lane;
+    gf* ps = p + s * PS + lane;
     const float uL = uring[2 * S + jc], uR = uring[3 * S + jc];
     const int lane_nm1 = (lane + N - 1) % P;
-    float up_ = us[(P - 1) * 64 + lane], uc = us[lane];
-    float vp_ = vs[(P - 1) * 64 + lane], vc = vs[lane];
-    for (int t = 0; t < P; ++t) {
-      const int t1 = t + 1 == P ? 0 : t + 1;
-      const float un = us[t1 * 64 + lane];
-      const float vn = vs[t1 * 64 + lane];
-      int c0 = t - lane;
-      c0 += c0 < 0 ? P : 0;  // column - 1 of this lane at slot t
-      int tl = t - rl;
-      tl += tl < 0 ? P : 0;  // column - 1 of the last lane
-      int tu = t + 63;
-      tu -= tu >= P ? P : 0;  // strip above, lane 63, column t+1
-      gcf* pdn = has_next ? vnext + tl * 64 : vring + S + min(tl, N - 1) + 1;
-      gcf* pup = has_prev ? vprev + tu * 64 + 63 : vring + min(t, N - 1) + 1;
-      const float vdn_e = *pdn, vup_e = *pup;
-      // neighbours (fluid.rs:342-344)
-      const float uplus = t == lane_nm1 ? uR : un;
-      const float uminus = t == lane ? uL : up_;
-      const float vdn_l = from_next_lane(vn);
-      const float vup_l = from_prev_lane(vp_);
-      const float vplus = last ? vdn_e : vdn_l;
-      const float vminus = first ? vup_e : vup_l;
-      const float d = -0.5f * (uplus - uminus + vplus - vminus) / sf;
-      if (row_ok && c0 < N) {
-        ds[t * 64 + lane] = d;
-        ps[t * 64 + lane] = 0.0f;
+    float um1 = us[(P - 1) * 64], vm1 = vs[(P - 1) * 64];  // slot before the block
+    for (int tb = 0; tb < P; tb += 16) {
+      float ub[17], vb[17], e[16];
+      const int tn = tb + 16 == P ? 0 : tb + 16;
+#pragma unroll
+      for (int k = 0; k < 17; ++k) {
+        const int t = k < 16 ? tb + k : tn;
+        ub[k] = us[t * 64];
+        vb[k] = vs[t * 64];
       }
-      up_ = uc;
-      uc = un;
-      vp_ = vc;
-      vc = vn;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) e[k] = eb[edge_index(tb + k, lane, rl, has_prev, has_next, N, P, S)];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int t = tb + k;
+        int c0 = t - lane;
+        c0 += c0 < 0 ? P : 0;
+        const float um = k == 0 ? um1 : ub[k - 1];
+        const float vm = k == 0 ? vm1 : vb[k - 1];
+        // neighbours (fluid.rs:342-344)
+        const float uplus = t == lane_nm1 ? uR : ub[k + 1];
+        const float uminus = t == lane ? uL : um;
+        // below: lane+1's value; the last lane gets e, its own (lane 63) or
+        // lane rl+1's (which loaded the value below the last row)
+        const float vplus = from_next_lane(lane == rl + 1 ? e[k] : vb[k + 1], e[k]);
+        const float vminus = from_prev_lane(vm, e[k]);  // lane 0: the value above the strip
+        const float d = -0.5f * (uplus - uminus + vplus - vminus) / sf;
+        if (row_ok && c0 < N) {
+          ds[t * 64] = d;
+          ps[t * 64] = 0.0f;
+        }
+      }
+      um1 = ub[15];
+      vm1 = vb[15];
     }
   }
   __syncthreads();
   bnd_lists(sc, M_PASSIVE, div, M_PASSIVE, p);
 }
 
-// project, second half (fluid.rs:364-374): subtract the pressure gradient,
-// then set_boundaries(AdjustRow, u) and (AdjustColumn, v).
+// project, second half (fluid.rs:364-374)
 __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__ u, gf* __restrict__ v,
                                            gcf* __restrict__ p) {
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
@@ -377,41 +413,42 @@ __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__
     const bool row_ok = j <= N;
     const int jc = min(j, N);
     const int rl = min(63, N - 1 - 64 * s);
-    const bool first = lane == 0, last = lane == rl;
     const bool has_prev = s > 0, has_next = s < ns - 1;
-    gcf* ps = p + s * PS;
-    gcf* pnext = p + (has_next ? s + 1 : s) * PS;
-    gcf* pprev = p + (has_prev ? s - 1 : s) * PS;
-    gf* us = u + s * PS;
-    gf* vs = v + s * PS;
+    gcf* ps = p + s * PS + lane;
+    gcf* eb = lane == 0 ? (has_prev ? p + (s - 1) * PS + 63 : pring) : (has_next ? p + (s + 1) * PS : pring);
+    gf* us = u + s * PS + lane;
+    gf* vs = v + s * PS + lane;
     const float pL = pring[2 * S + jc], pR = pring[3 * S + jc];
     const int lane_nm1 = (lane + N - 1) % P;
-    float pp = ps[(P - 1) * 64 + lane], pc = ps[lane];
-    for (int t = 0; t < P; ++t) {
-      const int t1 = t + 1 == P ? 0 : t + 1;
-      const float pn = ps[t1 * 64 + lane];
-      int c0 = t - lane;
-      c0 += c0 < 0 ? P : 0;
-      int tl = t - rl;
-      tl += tl < 0 ? P : 0;
-      int tu = t + 63;
-      tu -= tu >= P ? P : 0;
-      gcf* qdn = has_next ? pnext + tl * 64 : pring + S + min(tl, N - 1) + 1;
-      gcf* qup = has_prev ? pprev + tu * 64 + 63 : pring + min(t, N - 1) + 1;
-      const float pdn_e = *qdn, pup_e = *qup;
-      const float pplus = t == lane_nm1 ? pR : pn;
-      const float pminus = t == lane ? pL : pp;
-      const float pdn_l = from_next_lane(pn);
-      const float pup_l = from_prev_lane(pp);
-      const float pdn = last ? pdn_e : pdn_l;
-      const float pup = first ? pup_e : pup_l;
-      if (row_ok && c0 < N) {
-        const int o = t * 64 + lane;
-        us[o] = us[o] - 0.5f * (pplus - pminus) * sf;
-        vs[o] = vs[o] - 0.5f * (pdn - pup) * sf;
+    float pm1 = ps[(P - 1) * 64];
+    for (int tb = 0; tb < P; tb += 8) {  // 8 slots keep the block within 128 registers
+      float pb[9], e[8], ub[8], vb[8];
+      const int tn = tb + 8 == P ? 0 : tb + 8;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) pb[k] = ps[(k < 8 ? tb + k : tn) * 64];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int t = tb + k;
+        e[k] = eb[edge_index(t, lane, rl, has_prev, has_next, N, P, S)];
+        ub[k] = us[t * 64];
+        vb[k] = vs[t * 64];
       }
-      pp = pc;
-      pc = pn;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int t = tb + k;
+        int c0 = t - lane;
+        c0 += c0 < 0 ? P : 0;
+        const float pm = k == 0 ? pm1 : pb[k - 1];
+        const float pplus = t == lane_nm1 ? pR : pb[k + 1];
+        const float pminus = t == lane ? pL : pm;
+        const float pdn = from_next_lane(lane == rl + 1 ? e[k] : pb[k + 1], e[k]);
+        const float pup = from_prev_lane(pm, e[k]);
+        if (row_ok && c0 < N) {
+          us[t * 64] = ub[k] - 0.5f * (pplus - pminus) * sf;
+          vs[t * 64] = vb[k] - 0.5f * (pdn - pup) * sf;
+        }
+      }
+      pm1 = pb[7];
     }
   }
   __syncthreads();
@@ -470,43 +507,60 @@ __device__ __forceinline__ void advect_phase(const SceneDesc& sc, int mode0, gf*
     const int j = 64 * s + lane + 1;
     const bool row_ok = j <= N;
     const int jc = min(j, N);
-    gcf* us = u + s * PS;
-    gcf* vs = v + s * PS;
+    gcf* us = u + s * PS + lane;
+    gcf* vs = v + s * PS + lane;
     // pass A: first trigger column of this row
     int trig = N + 1;
-    for (int t = 0; t < P; ++t) {
-      int c0 = t - lane;
-      c0 += c0 < 0 ? P : 0;
-      if (c0 < N) {
-        const Trace r = back_trace(c0 + 1, jc, us[t * 64 + lane], vs[t * 64 + lane], dt0, sf, sfm1, S);
-        if (r.trig && c0 + 1 < trig) trig = c0 + 1;
+    for (int tb = 0; tb < P; tb += 16) {
+      float ub[16], vb[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        ub[k] = us[(tb + k) * 64];
+        vb[k] = vs[(tb + k) * 64];
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        int c0 = tb + k - lane;
+        c0 += c0 < 0 ? P : 0;
+        const Trace r = back_trace(c0 + 1, jc, ub[k], vb[k], dt0, sf, sfm1, S);
+        if (c0 < N && r.trig && c0 + 1 < trig) trig = c0 + 1;
       }
     }
     // pass B
     const int ro = ns * PS;
     const float left0 = d0out[ro + 2 * S + jc];
     const float left1 = two ? d1out[ro + 2 * S + jc] : 0.0f;
-    gf* o0 = d0out + s * PS;
-    gf* o1 = two ? d1out + s * PS : d0out;
-    for (int t = 0; t < P; ++t) {
-      int c0 = t - lane;
-      c0 += c0 < 0 ? P : 0;
-      const int c = c0 + 1;
-      if (row_ok && c <= N && c <= trig) {
-        const int o = t * 64 + lane;
-        if (c == trig && c == 1) {  // trigger in column 1: copy the ring cell (fluid.rs:421)
-          o0[o] = left0;
-          if (two) o1[o] = left1;
-        } else {
-          // c < trig: this cell; c == trig > 1: the left neighbour's value
-          const int tp = t == 0 ? P - 1 : t - 1;
-          const int tt = c == trig ? tp : t;
-          const int cc = c == trig ? c - 1 : c;
-          const Trace r = back_trace(cc, j, us[tt * 64 + lane], vs[tt * 64 + lane], dt0, sf, sfm1, S);
-          o0[o] = gather(d0in, r, S, P, ns);
-          if (two) o1[o] = gather(d1in, r, S, P, ns);
+    gf* o0 = d0out + s * PS + lane;
+    gf* o1 = (two ? d1out : d0out) + s * PS + lane;
+    float um1 = us[(P - 1) * 64], vm1 = vs[(P - 1) * 64];
+    for (int tb = 0; tb < P; tb += 8) {  // 8 slots: velocity loads, then 32-64 gathers in flight
+      float ub[8], vb[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        ub[k] = us[(tb + k) * 64];
+        vb[k] = vs[(tb + k) * 64];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int t = tb + k;
+        int c0 = t - lane;
+        c0 += c0 < 0 ? P : 0;
+        const int c = c0 + 1;
+        // c < trig: this cell; c == trig > 1: the left neighbour's value
+        const bool at = c == trig;
+        const float uu = at ? (k == 0 ? um1 : ub[k - 1]) : ub[k];
+        const float vv = at ? (k == 0 ? vm1 : vb[k - 1]) : vb[k];
+        const Trace tr = back_trace(at ? c - 1 : c, j, uu, vv, dt0, sf, sfm1, S);
+        const float g0 = gather(d0in, tr, S, P, ns);
+        const float g1 = two ? gather(d1in, tr, S, P, ns) : 0.0f;
+        if (row_ok && c <= N && c <= trig) {
+          const bool ring1 = at && c == 1;  // trigger in column 1: copy the ring cell (fluid.rs:421)
+          o0[t * 64] = ring1 ? left0 : g0;
+          if (two) o1[t * 64] = ring1 ? left1 : g1;
         }
       }
+      um1 = ub[7];
+      vm1 = vb[7];
     }
   }
   __syncthreads();
@@ -516,23 +570,40 @@ __device__ __forceinline__ void advect_phase(const SceneDesc& sc, int mode0, gf*
 // Frame sources: add_density (fluid.rs:122-123) and add_velocity
 // (fluid.rs:129-130) applied to every cell, interior and ring.
 __device__ __forceinline__ void add_sources_phase(const SceneDesc& sc, gf* f) {
-  const int n = (int)sc.fstride;
-  gcf* sd = (gcf*)sc.src;
-  gcf* su = sd + n;
-  gcf* sv = sd + 2 * n;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const float a = sd[e];
-    f[F_D * n + e] += a;
-    f[F_SC * n + e] += a;
-    f[F_U * n + e] += su[e];
-    f[F_V * n + e] += sv[e];
+  const int n4 = (int)(sc.fstride / 4);  // fstride is a multiple of 64
+  gcf4* sd = (gcf4*)sc.src;
+  gcf4* su = sd + n4;
+  gcf4* sv = sd + 2 * n4;
+  gf4* fd = (gf4*)f + F_D * n4;
+  gf4* fs = (gf4*)f + F_SC * n4;
+  gf4* fu = (gf4*)f + F_U * n4;
+  gf4* fv = (gf4*)f + F_V * n4;
+  for (int e = threadIdx.x; e < n4; e += blockDim.x) {
+    const f4 a = sd[e], b = su[e], c = sv[e];
+    fd[e] = fd[e] + a;
+    fs[e] = fs[e] + a;
+    fu[e] = fu[e] + b;
+    fv[e] = fv[e] + c;
   }
   __syncthreads();
 }
 
+// dst = src over the whole field (fstride is a multiple of 64 floats),
+// 8 float4 loads in flight per thread.
 __device__ __forceinline__ void copy_phase(const SceneDesc& sc, gf* __restrict__ dst, gcf* __restrict__ src) {
-  const int n = (int)sc.fstride;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
+  const int n4 = (int)(sc.fstride / 4);
+  gcf4* s4 = (gcf4*)src;
+  gf4* d4 = (gf4*)dst;
+  const int stride = blockDim.x;
+  int e = threadIdx.x;
+  for (; e + 7 * stride < n4; e += 8 * stride) {
+    f4 r[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = s4[e + k * stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d4[e + k * stride] = r[k];
+  }
+  for (; e < n4; e += stride) d4[e] = s4[e];
   __syncthreads();
 }
 
@@ -600,5 +671,5 @@ __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* 
   for (int fr = 0; fr < nframes; ++fr) eqf::frame_cyclic<D, B>(sc);
 }
 
-template __global__ void eqf_frames_cyclic<8, 16, 1024>(const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<8, 16, 512>(const eqf::SceneDesc* __restrict__, int);
+template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>(const eqf::SceneDesc* __restrict__, int);
+template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>(const eqf::SceneDesc* __restrict__, int);
