@@ -29,7 +29,6 @@ struct eqf_scene {
   int2* d_bnd = nullptr;
   size_t bnd_cap = 0;
   int nbnd[3] = {0, 0, 0};
-  float* d_hraw = nullptr;
   void* d_main = nullptr;     // one allocation: fields, codes, rowf, hand-off rows, descriptor
   float* d_stage = nullptr;   // row-major staging for uploads/downloads (lazy)
   uint8_t* d_mask = nullptr;  // mask + scratch for the boundary build (lazy)
@@ -43,7 +42,9 @@ struct eqf_scene {
 
 // Runtime services shared with the server (eqf_runtime.hip).
 int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
-hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int maxns, int nframes, hipStream_t st);
+// maxns / lds_floats: the largest strip count and cyc_lds_floats() of the scenes.
+hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int maxns, int lds_floats, int nframes,
+                               hipStream_t st);
 int eqf_internal_error(int code, const std::string& msg);
 int eqf_internal_check_params(const eqf_params* p);
 // eqf_scene_create on a caller-owned stream (the server's admission stream).

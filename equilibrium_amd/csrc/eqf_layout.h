@@ -55,7 +55,6 @@ struct SceneDesc {
   const uint8_t* code;   // boundary codes, interior cells only (same indexing as fields)
   const uint8_t* rowf;   // [S] row j holds a NoWall cell
   const int2* bnd;       // (dst, src) field offsets: ROW list, COL list, PASSIVE list
-  float* hraw;           // cyclic: 2*ns rows of P hand-off values between strips (see gs_wavefront)
   int64_t fstride;       // floats per field
   int S, N, ns;          // size, interior size, strips
   int P;                 // cyclic period (N rounded up to a multiple of 16)
@@ -81,6 +80,12 @@ struct SceneDesc {
 #endif
 
 EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
+
+// Exact mode keeps the strips' hand-off rows in LDS: 2*ns rows of P floats
+// (top ring, raw last rows, post-boundary first rows, bottom ring) plus a
+// scratch slot per lane that absorbs the stores of lanes with nothing to hand
+// off (33 floats apart, so the 64 lanes' stores fall in distinct banks).
+EQF_HD inline int cyc_lds_floats(int ns, int P) { return 2 * ns * P + 64 * 33; }
 
 // Ring offset inside a cyclic field.
 EQF_HD inline int64_t cyc_ring(int P, int ns) { return (int64_t)ns * P * 64; }

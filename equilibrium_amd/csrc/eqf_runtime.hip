@@ -122,7 +122,6 @@ static void fill_desc(eqf_scene* s) {
   d.code = s->d_code;
   d.rowf = s->d_rowf;
   d.bnd = s->d_bnd;
-  d.hraw = s->d_hraw;
   d.fstride = s->fstride;
   d.S = s->S;
   d.N = s->N;
@@ -234,15 +233,14 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t b_fld = up(sizeof(float) * NFIELDS * (size_t)s->fstride);
   const size_t b_code = up(ncode), b_rowf = up((size_t)s->S);
-  const size_t b_hraw = up(sizeof(float) * (2 * (size_t)s->ns * s->P + 4)), b_desc = up(sizeof(SceneDesc));
-  e = hipMalloc(&s->d_main, b_fld + b_code + b_rowf + b_hraw + b_desc);
+  const size_t b_desc = up(sizeof(SceneDesc));
+  e = hipMalloc(&s->d_main, b_fld + b_code + b_rowf + b_desc);
   if (e == hipSuccess) {
     char* base = (char*)s->d_main;
     s->d_fld = (float*)base;
     s->d_code = (uint8_t*)(base + b_fld);
     s->d_rowf = (uint8_t*)(base + b_fld + b_code);
-    s->d_hraw = (float*)(base + b_fld + b_code + b_rowf);
-    s->d_desc = (SceneDesc*)(base + b_fld + b_code + b_rowf + b_hraw);
+    s->d_desc = (SceneDesc*)(base + b_fld + b_code + b_rowf);
     e = hipMemsetAsync(s->d_fld, 0, sizeof(float) * NFIELDS * s->fstride, s->stream);
   }
   if (e != hipSuccess) {
@@ -490,7 +488,8 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
   const int W = waves_for(s->ns);
   while (nframes > 0) {
     const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
-    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(1), dim3(64 * W), 0, s->stream, s->d_desc, nf);
+    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(1), dim3(64 * W),
+                       sizeof(float) * cyc_lds_floats(s->ns, s->P), s->stream, s->d_desc, nf);
     HIPCHK(hipGetLastError());
     nframes -= nf;
   }
@@ -571,12 +570,15 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
   return EQF_OK;
 }
 
-hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int maxns, int nframes, hipStream_t st) {
+hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int maxns, int lds_floats, int nframes,
+                               hipStream_t st) {
   const int W = waves_for(maxns);
+  const size_t lds = sizeof(float) * (size_t)lds_floats;
   if (W <= 8)
-    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
+    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
   else
-    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(n), dim3(64 * W), 0, st, d_descs, nframes);
+    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
+                       nframes);
   return hipGetLastError();
 }
 
@@ -585,9 +587,12 @@ int eqf_internal_error(int code, const std::string& msg) { return fail(code, msg
 int eqf_internal_check_params(const eqf_params* p) { return check_params(p); }
 
 static hipError_t batch_launch(eqf_batch* b, int nframes_per_launch) {
-  int maxns = 1;
-  for (auto* s : b->scenes) maxns = std::max(maxns, s->ns);
-  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), maxns, nframes_per_launch, b->stream);
+  int maxns = 1, lds = 0;
+  for (auto* s : b->scenes) {
+    maxns = std::max(maxns, s->ns);
+    lds = std::max(lds, cyc_lds_floats(s->ns, s->P));
+  }
+  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), maxns, lds, nframes_per_launch, b->stream);
 }
 
 extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {

@@ -84,8 +84,7 @@ struct eqf_server {
 static uint64_t estimate_bytes(const eqf_params& p) {
   const int S = (int)p.size, N = S - 2, ns = (N + 63) / 64, P = cyc_period(N);
   const uint64_t fs = (uint64_t)cyc_field_floats(S, P, ns);
-  return (NFIELDS + 3) * fs * 4 + (uint64_t)ns * P * 64 + (uint64_t)S + (uint64_t)N * 96 + 2ull * ns * P * 4 +
-         sizeof(SceneDesc) + 4096;
+  return (NFIELDS + 3) * fs * 4 + (uint64_t)ns * P * 64 + (uint64_t)S + (uint64_t)N * 96 + sizeof(SceneDesc) + 4096;
 }
 
 static int admit(eqf_server* sv, Req* r, std::string& err) {
@@ -176,8 +175,12 @@ static void server_loop(eqf_server* sv) {
                              sv->stream);
         last_list = act;
       }
-      for (Req* r : act) maxns = std::max(maxns, r->scene->ns);
-      hipError_t e = eqf_internal_launch(sv->d_descs, (int)act.size(), maxns, 1, sv->stream);
+      int lds = 0;
+      for (Req* r : act) {
+        maxns = std::max(maxns, r->scene->ns);
+        lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
+      }
+      hipError_t e = eqf_internal_launch(sv->d_descs, (int)act.size(), maxns, lds, 1, sv->stream);
       if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->thread_error = hipGetErrorString(e);

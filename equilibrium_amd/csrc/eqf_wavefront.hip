@@ -37,6 +37,7 @@ typedef GAS float gf;
 typedef GAS const float gcf;
 typedef GAS const uint8_t gcu8;
 typedef GAS const int gci;
+typedef __attribute__((address_space(3))) float lf;  // LDS
 typedef float f4 __attribute__((ext_vector_type(4)));  // elementwise IEEE adds, no contraction
 typedef GAS f4 gf4;
 typedef GAS const f4 gcf4;
@@ -143,8 +144,8 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 // per-lane conditions of the block (active steps, first/last column) are
 // bit masks and step indices computed once per block.
 //
-// Strips hand values to each other through H (2*ns rows of P floats, indexed
-// by the READER's slot so reads need no wrap):
+// Strips hand values to each other through H, in LDS (2*ns rows of P floats,
+// indexed by the READER's slot so reads need no wrap):
 //   row 0            top ring values for strip 0's lane 0 (copied at entry)
 //   row 1+s          raw last-row values of strip s, read by lane 0 of s+1
 //   row ns-1+s       post-boundary row-0 values of strip s, read by the last
@@ -152,9 +153,10 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 //   row 2ns-1        bottom ring values for the last strip's last lane
 //
 // Ordering argument.  A value stored at global step g is read by another
-// wave through a load issued at a step >= g + B (a barrier lies in between;
-// on gfx950 the waves of a workgroup share one in-order vector-memory path,
-// so a load issued after the barrier observes stores issued before it).
+// wave through a load issued at a step >= g + B (a barrier lies in between:
+// LDS hand-offs complete before it, and on gfx950 the waves of a workgroup
+// share one in-order vector-memory path, so a global load issued after the
+// barrier observes global stores issued before it).
 // With L >= 64+B+D: strip s+1 lane 0 reads strip s's raw last-row value of a
 // column L-63-D >= B steps after it was stored; strip s's last lane reads
 // strip s+1's post-boundary row-0 value (and the AdjustColumn fix-up) of the
@@ -168,8 +170,8 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 // cell being updated, and the ring itself is written once after the last
 // sweep by the list pass.
 template <int MODE, int D, int B>
-__device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict__ X, gcf* __restrict__ X0,
-                                             float a, float cr, int iters) {
+__device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict__ H, gf* __restrict__ X,
+                                             gcf* __restrict__ X0, float a, float cr, int iters) {
   static_assert(16 % B == 0 && D < B, "B must divide 16 and exceed the prefetch depth");
   constexpr int L = (64 + B + D + B - 1) / B * B;  // strip lag, a multiple of B
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
@@ -177,7 +179,6 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict
   const int wave = uni(threadIdx.x >> 6);
   const int RO = ns * P * 64;
   gf* __restrict__ ring = X + RO;
-  gf* __restrict__ H = (gf*)sc.hraw;
   const int G = (ns - 1) * L + 64 + iters * P + 1;
   const int Gr = (G + B - 1) / B * B;
 
@@ -229,7 +230,6 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict
   const rsrc_t rX = make_rsrc(X, fbytes);
   const rsrc_t rX0 = make_rsrc(X0, fbytes);
   const rsrc_t rC = make_rsrc((gcu8*)sc.code, (uint32_t)RO);
-  const rsrc_t rH = make_rsrc(H, (uint32_t)(2 * ns * P) * 4u);
   const int OOB = (int)0x80000000;  // voffset that drops a store / reads 0
   const int vl = lane * 4;
   const int vls = valid ? vl : OOB;                   // stores: rows past N stay untouched
@@ -240,11 +240,12 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict
   const bool bot = last && !has_next;                 // down neighbour is the bottom ring
   const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
   const int dn_row = has_next ? ns + s : 2 * ns - 1;  // post row of the strip below, or the bottom ring
-  const int ve = first ? up_row * P * 4 : (last_e ? dn_row * P * 4 : OOB);
-  const int hraw_base = uni((1 + s) * P * 4);              // this strip's raw last row (if has_next)
-  const int hpost_base = uni((ns - 1 + s) * P * 4);        // this strip's post row 0 (if has_prev)
-  const int vhraw = (last && valid && has_next) ? 0 : OOB;
-  const int vhpost = (first && has_prev) ? 0 : OOB;
+  const int e_row = (last_e ? dn_row : up_row) * P;  // H row of this lane's edge value (floats)
+  const int TR = 2 * ns * P + lane * 33;              // this lane's scratch slot in LDS
+  const int raw_row = (1 + s) * P;                    // this strip's raw last row (if has_next)
+  const int post_row = (ns - 1 + s) * P;              // this strip's post row 0 (if has_prev)
+  const bool raw_ok = last && valid && has_next;      // lane 63 hands its raw values down
+  const bool post_ok = first && has_prev;             // lane 0 hands its final values up
   const int sb = uni(s * P * 256);                        // strip base in X (bytes)
   const int cb = uni(s * P * 64);                        // strip base in the code array (bytes)
   const int fxb = uni(has_prev ? sb - P * 256 + 252 : 0);  // lane 63 of the strip above (bytes)
@@ -280,7 +281,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, gf* __restrict
       fxs[i] = ldf(rX, vl + (i + 1) * 256, sb);
       fx0[i] = ldf(rX0, vl + i * 256, sb);
       fcode[i] = ldu8(rC, lane + i * 64, cb);
-      fe[i] = ldf(rH, ve + i * 4, 0);
+      fe[i] = H[e_row + i];
     }
 #pragma nounroll
     while (blk < nblk) {
@@ -615,7 +616,7 @@ enum PhaseKind { PK_GS, PK_DIV, PK_GRAD, PK_ADV, PK_COPY };
 // phases run from a loop so that each phase kernel body exists once in the
 // code object.
 template <int D, int B>
-__device__ __forceinline__ void frame_cyclic(const SceneDesc& sc) {
+__device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
   gf* f = (gf*)sc.fld;
   const int n = (int)sc.fstride;
   gf* Dn = f + F_D * n;
@@ -646,9 +647,9 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc) {
     }
     if (kind == PK_GS) {
       if (iters > 0) {
-        if (mode == M_ROW) gs_wavefront<M_ROW, D, B>(sc, x, y, a, cr, iters);
-        else if (mode == M_COL) gs_wavefront<M_COL, D, B>(sc, x, y, a, cr, iters);
-        else gs_wavefront<M_PASSIVE, D, B>(sc, x, y, a, cr, iters);
+        if (mode == M_ROW) gs_wavefront<M_ROW, D, B>(sc, H, x, y, a, cr, iters);
+        else if (mode == M_COL) gs_wavefront<M_COL, D, B>(sc, H, x, y, a, cr, iters);
+        else gs_wavefront<M_PASSIVE, D, B>(sc, H, x, y, a, cr, iters);
       }
     } else if (kind == PK_DIV) {
       div_phase(sc, x, y, z, w);
@@ -667,8 +668,9 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc) {
 // grid = number of scenes, block = 64 * waves; scene b is descs[b].
 template <int D, int B, int MAXT>
 __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes) {
+  extern __shared__ float eqf_lds[];  // cyc_lds_floats(ns, P) floats, sized by the launch
   const eqf::SceneDesc& sc = descs[blockIdx.x];
-  for (int fr = 0; fr < nframes; ++fr) eqf::frame_cyclic<D, B>(sc);
+  for (int fr = 0; fr < nframes; ++fr) eqf::frame_cyclic<D, B>(sc, (eqf::lf*)eqf_lds);
 }
 
 template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>(const eqf::SceneDesc* __restrict__, int);
