@@ -52,7 +52,7 @@ enum CodeBits : uint32_t {
 struct SceneDesc {
   float* fld;            // NFIELDS fields, `fstride` floats apart
   const float* src;      // per-frame sources (density, vx, vy) or nullptr
-  const uint8_t* code;   // boundary codes, interior cells only (same indexing as fields)
+  const uint8_t* code;   // boundary codes, interior cells only (cyclic: cyc_code_off; row-major: x + y*S)
   const uint8_t* rowf;   // [S] row j holds a NoWall cell
   const int2* bnd;       // (dst, src) field offsets: ROW list, COL list, PASSIVE list
   int64_t fstride;       // floats per field
@@ -101,6 +101,17 @@ EQF_HD inline int64_t cyc_off(int x, int y, int S, int P, int ns) {
   int t = x - 1 + r;
   if (t >= P) t -= P;
   return ((int64_t)s * P + t) * 64 + r;
+}
+
+// Boundary codes of a cyclic scene: one byte per interior cell, with the
+// four slots 4k..4k+3 of a lane packed in one 32-bit word, so a wave reads
+// the codes of four wavefront steps with one load: slot t of lane r in strip
+// s is byte (s*P + (t & ~3))*64 + 4*r + (t & 3).
+EQF_HD inline int64_t cyc_code_off(int x, int y, int P) {
+  const int j = y - 1, s = j >> 6, r = j & 63;
+  int t = x - 1 + r;
+  if (t >= P) t -= P;
+  return ((int64_t)s * P + (t & ~3)) * 64 + 4 * r + (t & 3);
 }
 
 EQF_HD inline int64_t cyc_field_floats(int S, int P, int ns) {

@@ -106,7 +106,7 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (colf[x]) c |= C_COLF;
     if (g.layout == L_CYCLIC && ((y - 1) & 63) == 0 && y > 1 && !wall(x, y - 1) && wall(x, y) && !wall(x, y - 2))
       c |= C_FIXUP;
-    if (fill) code[off_of(g, x, y)] = (uint8_t)c;
+    if (fill) code[g.layout == L_CYCLIC ? cyc_code_off(x, y, g.P) : off_of(g, x, y)] = (uint8_t)c;
     if (fl) {
       int sx = -1;
       if (wall(x + 1, y)) sx = x + 1;

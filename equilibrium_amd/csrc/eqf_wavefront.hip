@@ -73,8 +73,8 @@ __device__ __forceinline__ rsrc_t make_rsrc(GAS const void* base, uint32_t bytes
 __device__ __forceinline__ float ldf(rsrc_t r, int voff, int soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
-__device__ __forceinline__ uint32_t ldu8(rsrc_t r, int voff, int soff) {
-  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0);
+__device__ __forceinline__ uint32_t ldu32(rsrc_t r, int voff, int soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
 }
 __device__ __forceinline__ void stf(float v, rsrc_t r, int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
@@ -235,7 +235,8 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int vls = valid ? vl : OOB;                   // stores: rows past N stay untouched
   const bool last_e = last && lane == 63;             // down neighbour from H
   const bool mir = !has_next && rl < 63 && lane == rl + 1;  // holds the mirrored bottom ring
-  const int vmir = (last && !has_next && rl < 63) ? (rl + 1) * 4 : OOB;  // Passive: refresh the mirror
+  const bool has_mirror = !has_next && rl < 63;              // (uniform) the strip has a mirrored row
+  const int vmir = (last && has_mirror) ? (rl + 1) * 4 : OOB;  // Passive: lane rl refreshes the mirror
   const bool top = first && !has_prev;                // up neighbour is the top ring
   const bool bot = last && !has_next;                 // down neighbour is the bottom ring
   const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
@@ -253,7 +254,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int o64 = uni(64 % P);
 
   float fxs[D], fx0[D], fe[D];
-  uint32_t fcode[D];
+  uint32_t fcw[4];  // packed codes of steps 4k..4k+3 in fcw[k & 3], loaded 13-16 steps ahead
 
   // Strip s idles for its first s*L/B blocks (every lane before its first
   // sweep): those blocks are barriers only.  Its first working block then
@@ -280,9 +281,10 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
     for (int i = 0; i < D; ++i) {  // prefetch the first D steps (slot i)
       fxs[i] = ldf(rX, vl + (i + 1) * 256, sb);
       fx0[i] = ldf(rX0, vl + i * 256, sb);
-      fcode[i] = ldu8(rC, lane + i * 64, cb);
       fe[i] = H[e_row + i];
     }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fcw[k] = ldu32(rC, vl + k * 256, cb);  // slots 0..15
 #pragma nounroll
     while (blk < nblk) {
       if (tau >= 64 && tau + B <= itP) {
