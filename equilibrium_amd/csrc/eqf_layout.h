@@ -36,7 +36,8 @@ enum Field { F_D = 0, F_SC = 1, F_U = 2, F_V = 3, F_U0 = 4, F_V0 = 5, NFIELDS = 
 enum Mode { M_ROW = 0, M_COL = 1, M_PASSIVE = 2 };  // Orientation, fluid.rs:22-29
 enum Layout { L_CYCLIC = 0, L_ROWMAJOR = 1 };
 
-// Per-cell boundary code (one byte per interior cell, laid out like a field).
+// Per-cell boundary code of a ROWMAJOR scene (one byte per interior cell,
+// laid out like a field).
 enum CodeBits : uint32_t {
   C_FLUID = 1u,     // cell is ContainerWall::NoWall
   C_LWALL = 2u,     // (c-1, j) is a wall
@@ -49,10 +50,21 @@ enum CodeBits : uint32_t {
   C_COLF = 64u,     // column c holds at least one NoWall cell (Passive ring copy)
 };
 
+// Boundary selectors of a CYCLIC scene: one byte per interior cell and mode
+// (array of mode m at byte m * ns * P * 64, cells at cyc_code_off), telling
+// the wavefront what set_boundaries of that mode makes of the cell:
+//   M_ROW:     0 keep, 1 -(left neighbour), 2 -(right neighbour)  (right wins, fluid.rs:151-164)
+//   M_COL:     0 keep, 1 -(upper neighbour), 2 -(lower neighbour) (up wins, fluid.rs:165-177);
+//              a lower neighbour in the next strip is left to SEL_FIXUP there;
+//              3 (SEL_FIXUP): the cell is a wall at the top row of a strip and
+//              the fluid cell above it takes -(this cell) (and nothing above that)
+//   M_PASSIVE: 1 if the column holds a NoWall cell (ring copy, fluid.rs:179-187)
+enum Sel : uint32_t { SEL_KEEP = 0, SEL_NEG1 = 1, SEL_NEG2 = 2, SEL_FIXUP = 3 };
+
 struct SceneDesc {
   float* fld;            // NFIELDS fields, `fstride` floats apart
   const float* src;      // per-frame sources (density, vx, vy) or nullptr
-  const uint8_t* code;   // boundary codes, interior cells only (cyclic: cyc_code_off; row-major: x + y*S)
+  const uint8_t* code;   // cyclic: 3 selector arrays (Sel); row-major: CodeBits at x + y*S
   const uint8_t* rowf;   // [S] row j holds a NoWall cell
   const int2* bnd;       // (dst, src) field offsets: ROW list, COL list, PASSIVE list
   int64_t fstride;       // floats per field
@@ -103,10 +115,10 @@ EQF_HD inline int64_t cyc_off(int x, int y, int S, int P, int ns) {
   return ((int64_t)s * P + t) * 64 + r;
 }
 
-// Boundary codes of a cyclic scene: one byte per interior cell, with the
-// four slots 4k..4k+3 of a lane packed in one 32-bit word, so a wave reads
-// the codes of four wavefront steps with one load: slot t of lane r in strip
-// s is byte (s*P + (t & ~3))*64 + 4*r + (t & 3).
+// Boundary selectors of a cyclic scene: one byte per interior cell and mode,
+// with the four slots 4k..4k+3 of a lane packed in one 32-bit word, so a wave
+// reads the selectors of four wavefront steps with one load: slot t of lane r
+// in strip s is byte (s*P + (t & ~3))*64 + 4*r + (t & 3) of the mode's array.
 EQF_HD inline int64_t cyc_code_off(int x, int y, int P) {
   const int j = y - 1, s = j >> 6, r = j & 63;
   int t = x - 1 + r;

@@ -104,9 +104,21 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (wall(x, y + 1)) c |= C_DWALL;
     if (wall(x, y - 1)) c |= C_UWALL;
     if (colf[x]) c |= C_COLF;
-    if (g.layout == L_CYCLIC && ((y - 1) & 63) == 0 && y > 1 && !wall(x, y - 1) && wall(x, y) && !wall(x, y - 2))
-      c |= C_FIXUP;
-    if (fill) code[g.layout == L_CYCLIC ? cyc_code_off(x, y, g.P) : off_of(g, x, y)] = (uint8_t)c;
+    if (fill && g.layout == L_ROWMAJOR) code[off_of(g, x, y)] = (uint8_t)c;
+    if (fill && g.layout == L_CYCLIC) {
+      const int64_t o = cyc_code_off(x, y, g.P), mstride = (int64_t)g.ns * g.P * 64;
+      const bool strip_bottom = (y & 63) == 0 && y < N;  // the row below is in the next strip
+      uint8_t srow = SEL_KEEP, scol = SEL_KEEP;
+      if (fl) {
+        srow = (c & C_RWALL) ? SEL_NEG2 : (c & C_LWALL) ? SEL_NEG1 : SEL_KEEP;
+        scol = (c & C_UWALL) ? SEL_NEG1 : ((c & C_DWALL) && !strip_bottom) ? SEL_NEG2 : SEL_KEEP;
+      } else if (((y - 1) & 63) == 0 && y > 1 && !wall(x, y - 1) && !wall(x, y - 2)) {
+        scol = SEL_FIXUP;
+      }
+      code[o] = srow;
+      code[mstride + o] = scol;
+      code[2 * mstride + o] = colf[x] ? 1 : 0;
+    }
     if (fl) {
       int sx = -1;
       if (wall(x + 1, y)) sx = x + 1;
@@ -230,7 +242,7 @@ int eqf_dev_rebuild_bnd(eqf_scene* s) {
   }
   e = hipMemsetAsync(d_cnt, 0, 3 * sizeof(int), s->stream);
   if (e == hipSuccess) {
-    const size_t ncode = s->layout == L_CYCLIC ? (size_t)s->ns * s->P * 64 : (size_t)n;
+    const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)s->ns * s->P * 64 : (size_t)n;
     e = hipMemsetAsync(s->d_code, 0, ncode, s->stream);
   }
   if (e == hipSuccess) {

@@ -229,7 +229,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const uint32_t fbytes = (uint32_t)sc.fstride * 4u;
   const rsrc_t rX = make_rsrc(X, fbytes);
   const rsrc_t rX0 = make_rsrc(X0, fbytes);
-  const rsrc_t rC = make_rsrc((gcu8*)sc.code, (uint32_t)RO);
+  const rsrc_t rC = make_rsrc((gcu8*)sc.code + MODE * RO, (uint32_t)RO);  // this mode's selectors
   const int OOB = (int)0x80000000;  // voffset that drops a store / reads 0
   const int vl = lane * 4;
   const int vls = valid ? vl : OOB;                   // stores: rows past N stay untouched
@@ -253,7 +253,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int itP = uni(iters * P);
   const int o64 = uni(64 % P);
 
-  float fxs[D], fx0[D], fe[D];
+  float fxs[D], fx0[D], fe[D], fe2[D];  // fe / fe2: the edge value, as the up / down shift's edge operand
   uint32_t fcw[4];  // packed codes of steps 4k..4k+3 in fcw[k & 3], loaded 13-16 steps ahead
 
   // Strip s idles for its first s*L/B blocks (every lane before its first
@@ -282,6 +282,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
       fxs[i] = ldf(rX, vl + (i + 1) * 256, sb);
       fx0[i] = ldf(rX0, vl + i * 256, sb);
       fe[i] = H[e_row + i];
+      fe2[i] = H[e_row + i];
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) fcw[k] = ldu32(rC, vl + k * 256, cb);  // slots 0..15
