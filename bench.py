@@ -109,37 +109,43 @@ def bench_single_dropin(scs, frames):
     return wall
 
 
-def cpu_baseline(scs, frames, threads):
-    """The CPU oracle (C restatement of the reference step, -O2, no FMA) on a
-    sample of the same scenes, one scene per thread."""
+def cpu_baseline(scs, seconds, threads):
+    """The CPU oracle (C restatement of the reference step, -O2, no FMA) on the
+    same scenes, one scene per thread, whole frames of the mix until about
+    `seconds` of wall time have passed (at least one frame)."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
-    import oracle
+    import oracle  # noqa: F401
     from helpers import oracle_state
 
     states = [oracle_state(sc) for sc in scs]
 
     def run(i):
         sc = scs[i]
-        states[i].run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], frames,
+        states[i].run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], 1,
                       src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
 
     os.environ.setdefault("OMP_NUM_THREADS", "1")
+    frames = 0
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(threads) as ex:
-        list(ex.map(run, range(len(scs))))
-    wall = time.perf_counter() - t0
+        while True:
+            list(ex.map(run, range(len(scs))))
+            frames += 1
+            wall = time.perf_counter() - t0
+            if wall * (frames + 1) / frames > seconds:  # the next frame would overrun
+                break
     cells = sum((sc["size"] - 2) ** 2 for sc in scs) * frames
-    return cells / wall, wall
+    return cells / wall, wall, frames
 
 
 def bench_redblack_4096(steps, check_frames, N=4096):
-    """Config 2: one N=4096 grid, 40 sweeps, red-black mode with LDS temporal
+    """One N=4096 grid, 40 sweeps, red-black mode with LDS temporal
     blocking; autotuned tiling; accuracy vs the f64 red-black oracle on the
     first frames; pressure Poisson residual of the last projection."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from equilibrium_amd import tune
 
-    sc = scenes.config2(N=N)
+    sc = scenes.redblack_scene(N=N)
     t_setup = time.perf_counter()
     h = _abi.Scene(scenes.scene_params(sc, _abi.GS_REDBLACK))
     scenes.apply_to_scene(h, sc)
@@ -202,8 +208,8 @@ def bench_redblack_4096(steps, check_frames, N=4096):
 
 
 def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
-    """Config 4: Poisson arrivals of scene requests (N log-uniform in
-    [64, n_max], frames U(50,500), config-0 style sources and obstacles),
+    """Poisson arrivals of scene requests (N log-uniform in
+    [64, n_max], frames U(50,500), sources and obstacles as in small_scene),
     admitted against an HBM budget; each rate runs `seconds` of arrivals,
     then the remaining requests are cancelled.  p50/p99 frame latency =
     tick durations weighted by resident scenes.  To keep the client on the
@@ -215,7 +221,7 @@ def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
     logs = [math.log(N) for N in sizes]
     out = []
     for rate in rates:
-        reqs = scenes.config4_requests(rate, seconds, n_max=n_max)
+        reqs = scenes.serving_requests(rate, seconds, n_max=n_max)
         srv = _abi.Server(int(budget_gb * (1 << 30)))
         t0 = time.perf_counter()
         tickets = []
@@ -278,7 +284,7 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log("calibration copy: 3 x 1 GiB read + 1 GiB write (torch copy kernel)")
-    mix = scenes.config3(n=args.scenes, frames=args.steps)
+    mix = scenes.batched_mix(n=args.scenes, frames=args.steps)
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded scenes generated on the host)"}
@@ -348,7 +354,7 @@ def main():
 
     if "n1024" in legs:
         # N=1024 single grid, exact mode
-        c1 = scenes.config1()
+        c1 = scenes.single_1024()
         h = make_scenes([c1])[0]
         h.step(1)
         h.sync()
@@ -363,26 +369,19 @@ def main():
         log(f"N=1024 single: {1024 * 1024 * nf / w1:.4g} cell-updates/s")
 
     if "rb4096" in legs:
-        # config 2: N=4096 single grid, red-black mode
+        # N=4096 single grid, red-black mode
         out["n4096_single_redblack"] = bench_redblack_4096(args.rb_frames, args.rb_check_frames)
         log(f"N=4096 red-black: {out['n4096_single_redblack']['value']:.4g} cell-updates/s")
 
     if "serve" in legs:
-        # config 4: serving with Poisson arrivals
+        # serving with Poisson arrivals
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         out["serving"] = bench_serving(rates, args.serve_seconds)
 
     if "cpu" in legs:
         # CPU baseline: oracle on this host, bounded sample
         threads = max(1, min(16, os.cpu_count() or 1))
-        probe = mix[:threads]
-        _, wprobe = cpu_baseline(probe, 1, threads)
-        cells_probe = sum((sc["size"] - 2) ** 2 for sc in probe)
-        cells = sum((sc["size"] - 2) ** 2 for sc in mix)
-        # whole mix, as many frames as fit in about cpu_seconds
-        est_frame_s = wprobe * cells / max(cells_probe, 1)
-        cframes = int(max(1, min(20, round(args.cpu_seconds / max(est_frame_s, 1e-3)))))
-        rate, wcpu = cpu_baseline(mix, cframes, threads)
+        rate, wcpu, cframes = cpu_baseline(mix, args.cpu_seconds, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port",
                                "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) from the "

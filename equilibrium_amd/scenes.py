@@ -124,12 +124,12 @@ def stam_scene(N, seed, *, dt=0.1, diffusion=0.0, viscosity=1e-4, iters=20, fram
                 init_u=None, init_v=None)
 
 
-def config0(seed=None):
+def small_scene(seed=None):
     """N=128, dt 0.1, diffusion 0, viscosity 1e-4, 20 sweeps, 100 frames."""
     return stam_scene(128, derive_seed(0) if seed is None else seed)
 
 
-def config1(seed=None):
+def single_1024(seed=None):
     """N=1024, viscosity 1e-4, diffusion 1e-5, 20 sweeps, 200 frames,
     3 rectangles with sides U(N/20, N/8), 4 Gaussian density/velocity sources."""
     return stam_scene(1024, derive_seed(1) if seed is None else seed, diffusion=1e-5, frames=200,
@@ -151,7 +151,7 @@ def curl_noise(S, seed, smooth, peak):
     return (u * scale).astype(np.float32).ravel(), (v * scale).astype(np.float32).ravel()
 
 
-def config2(seed=None, N=4096):
+def redblack_scene(seed=None, N=4096):
     """N=4096, 40 sweeps, 100 frames, 16 rectangles, divergence-free curl-noise
     initial velocity with peak |u| = 2; dt 0.1, viscosity 1e-4, diffusion 1e-5
     (not fixed by the workload description); 4 Gaussian density sources."""
@@ -164,7 +164,7 @@ def config2(seed=None, N=4096):
     return sc
 
 
-def config3(seed=None, n=512, frames=200):
+def batched_mix(seed=None, n=512, frames=200):
     """Batched mix: N uniform in {64,128,256,512}, viscosity and diffusion
     log-uniform in [1e-6, 1e-3], 0-4 rectangles, 20 sweeps, dt 0.1."""
     base = derive_seed(3) if seed is None else seed
@@ -180,9 +180,9 @@ def config3(seed=None, n=512, frames=200):
     return out
 
 
-def config4_requests(rate, duration_s, seed=None, n_max=2048, n_min=64):
+def serving_requests(rate, duration_s, seed=None, n_max=2048, n_min=64):
     """Serving workload: Poisson arrivals at `rate` scenes/s over `duration_s`;
-    N = round(log-uniform[64, 2048]); frames U(50, 500); config-0 scenes."""
+    N = round(log-uniform[64, 2048]); frames U(50, 500); scenes like small_scene."""
     base = derive_seed(4, int(rate)) if seed is None else seed
     rng = SplitMix64(base)
     t = 0.0

@@ -74,12 +74,12 @@ def test_ctx_create_without_gpu_reports_device_error():
 
 
 def test_scene_generator_is_deterministic():
-    a = scenes.config3(n=6, frames=5)
-    b = scenes.config3(n=6, frames=5)
+    a = scenes.batched_mix(n=6, frames=5)
+    b = scenes.batched_mix(n=6, frames=5)
     for x, y in zip(a, b):
         assert x["size"] == y["size"] and x["rects"] == y["rects"]
         assert np.array_equal(x["src_d"], y["src_d"]) and np.array_equal(x["src_u"], y["src_u"])
-    sizes = {sc["size"] for sc in scenes.config3(n=64, frames=1)}
+    sizes = {sc["size"] for sc in scenes.batched_mix(n=64, frames=1)}
     assert sizes <= {66, 130, 258, 514} and len(sizes) == 4
 
 

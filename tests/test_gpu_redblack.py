@@ -69,7 +69,7 @@ def test_redblack_accuracy_vs_fp64():
     """N=254, 40 sweeps, curl-noise velocity, 8 frames: f32 GPU vs f64 CPU."""
     N = 254
     _case(f"rb fp64 N={N}")
-    sc = scenes.config2(N=N)
+    sc = scenes.redblack_scene(N=N)
     sc["frames"] = 8
     s = gpu_scene(sc, _abi.GS_REDBLACK)
     st = oracle_state(sc).astype(np.float64)

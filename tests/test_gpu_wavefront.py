@@ -96,27 +96,27 @@ def test_zero_iterations_and_default_state():
     assert_state_equal(download_all(s), st, "iters=0")
 
 
-def test_config0_full_bit_exact():
-    """Workload config 0 in full: N=128, 20 sweeps, 100 frames."""
-    sc = scenes.config0()
-    _case("config0 N=128 100 frames")
+def test_small_scene_full_bit_exact():
+    """The N=128 scene (20 sweeps) for all its 100 frames."""
+    sc = scenes.small_scene()
+    _case("small_scene N=128 100 frames")
     s = gpu_scene(sc)
     s.step(sc["frames"])
     s.sync()
     st = run_oracle(sc)
-    assert_state_equal(download_all(s), st, "config0")
+    assert_state_equal(download_all(s), st, "small_scene")
 
 
-def test_config1_first_frames_bit_exact():
-    """Workload config 1 (N=1024, 16 strips, 3 rectangles, 4 sources), first
+def test_single_1024_first_frames_bit_exact():
+    """The N=1024 scene (16 strips, 3 rectangles, 4 sources), first
     2 frames compared with the oracle."""
-    sc = scenes.config1()
-    _case("config1 N=1024 2 frames")
+    sc = scenes.single_1024()
+    _case("single_1024 N=1024 2 frames")
     s = gpu_scene(sc)
     s.step(2)
     s.sync()
     st = run_oracle(sc, frames=2)
-    assert_state_equal(download_all(s), st, "config1")
+    assert_state_equal(download_all(s), st, "single_1024")
 
 
 def test_step_in_pieces_equals_one_call():
