@@ -139,8 +139,8 @@ def cpu_baseline(scs, seconds, threads):
 
 
 def bench_redblack_4096(steps, check_frames, N=4096):
-    """One N=4096 grid, 40 sweeps, red-black mode with LDS temporal
-    blocking; autotuned tiling; accuracy vs the f64 red-black oracle on the
+    """One N=4096 grid, 40 sweeps, red-black mode with register temporal
+    blocking; autotuned band shape; accuracy vs the f64 red-black oracle on the
     first frames; pressure Poisson residual of the last projection."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from equilibrium_amd import tune
@@ -155,7 +155,7 @@ def bench_redblack_4096(steps, check_frames, N=4096):
     tile, tsw, timings = tune.tune_redblack(probe, frames=1)
     probe.close()
     h.set_tuning(tile, tsw)
-    log(f"rb N={N}: autotuned tile {tune.TILES[tile]} x {tsw} sweeps/block: {timings}")
+    log(f"rb N={N}: autotuned band {tune.TILES[tile]} x {tsw} sweeps/launch: {timings}")
     h.step(1)
     h.sync()
     t0 = time.perf_counter()
@@ -164,7 +164,7 @@ def bench_redblack_4096(steps, check_frames, N=4096):
     wall = time.perf_counter() - t0
     rate = N * N * steps / wall
     res = {"value": rate, "unit": "cell-updates/s", "ms_per_frame": wall * 1e3 / steps,
-           "frames_timed": steps, "tile": list(tune.TILES[tile]), "sweeps_per_lds_block": tsw,
+           "frames_timed": steps, "band_waves_rows": list(tune.TILES[tile]), "sweeps_per_launch": tsw,
            "gs_sweeps": sc["iters"], "alg_bytes_per_cell_update": b_alg(sc["iters"]),
            "roofline_frac": rate * b_alg(sc["iters"]) / (HBM_PEAK_GBS * 1e9)}
     h.close()

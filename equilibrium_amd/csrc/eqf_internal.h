@@ -42,8 +42,14 @@ struct eqf_scene {
 
 // Runtime services shared with the server (eqf_runtime.hip).
 int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
-// maxns / lds_floats: the largest strip count and cyc_lds_floats() of the scenes.
-hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int maxns, int lds_floats, int nframes,
+// Issue priority of a scene among the scenes of one launch: 3 for the largest
+// (strip count), then 2, 1, 0 for each halving.
+int eqf_batch_prio(int ns, int maxns);
+// Waves a wavefront-mode scene uses: its strips times the sweep streams per
+// strip (up to 16 waves, at most P/64 streams).
+int eqf_scene_waves(int ns, int P);
+// waves / lds_floats: the largest eqf_scene_waves() and cyc_lds_floats() of the scenes.
+hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int waves, int lds_floats, int nframes,
                                hipStream_t st);
 int eqf_internal_error(int code, const std::string& msg);
 int eqf_internal_check_params(const eqf_params* p);

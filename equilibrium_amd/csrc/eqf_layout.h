@@ -78,7 +78,7 @@ struct SceneDesc {
   float sf;              // (float)S           (fluid.rs:345, 396)
   float sfm1;            // (float)S - 1.0f    (fluid.rs:403)
   int has_src;
-  int pad_;
+  int prio;              // wave issue priority 0..3 in a batch launch (largest scenes highest)
 };
 
 // Exact-mode wavefront schedule: prefetch depth (steps) and steps between

@@ -3,13 +3,13 @@
 // Every lin_solve (fluid.rs:301-325) updates the (i+j)-even cells of a sweep
 // first and the odd cells second, then applies set_boundaries, so all cells
 // of one colour are independent and the whole grid is updated in parallel by
-// many workgroups.  Sweeps are temporally blocked in LDS: a workgroup loads
-// its TX x TY tile plus a halo of 3T cells, runs T full sweeps (red, black,
-// set_boundaries) in LDS, and writes the tile back, so x and x0 cross HBM
-// once per T sweeps instead of once per sweep.  Each full sweep shrinks the
-// valid region by 3 cells: red reads black neighbours (1), black reads the
-// new red neighbours (2), and set_boundaries copies a wall neighbour (3).
-// The tile pass reads one buffer and writes another (ping-pong).
+// many workgroups.  Sweeps are temporally blocked in registers (k_rb_band):
+// a workgroup loads a region (its output tile plus a halo of 3T cells), runs
+// T full sweeps (red, black, set_boundaries) on it, and writes the tile back,
+// so x and x0 cross HBM once per T sweeps instead of once per sweep.  Each
+// full sweep shrinks the valid region by 3 cells: red reads black neighbours
+// (1), black reads the new red neighbours (2), and set_boundaries copies a
+// wall neighbour (3).  A pass reads one buffer and writes another (ping-pong).
 //
 // The other phases (divergence, gradient, advection with its row `break`,
 // set_boundaries lists, copy) are grid-wide kernels.  One frame is captured
@@ -38,96 +38,193 @@ __device__ __forceinline__ int sat_idx(float f, int S) {
   return (int)f;
 }
 
-// Red-black Gauss-Seidel, T full sweeps per launch on a TX x TY tile.
-template <int MODE, int T, int TX, int TY>
-__global__ void __launch_bounds__(256) k_rb_gs(const SceneDesc* __restrict__ dp, const float* __restrict__ src,
-                                               float* __restrict__ dst, const float* __restrict__ x0g, float a,
-                                               float cr, int nsweeps) {
-  constexpr int H = 3 * T;
-  constexpr int RW = TX + 2 * H, RH = TY + 2 * H;
-  constexpr int P = RW + 1;  // row pitch in LDS
-  __shared__ float sx[RH * P];
-  __shared__ float s0[RH * P];
-  __shared__ uint8_t scode[RH * P];
+// Lane shifts (DPP wave_shr:1 / wave_shl:1); the lane without a source reads
+// 0 (bound_ctrl).  The empty asm keeps each shift where it is written,
+// executed by the whole wave (a shift moved under a divergent branch would
+// read disabled lanes), and keeps selects on its result from being folded
+// into the move.
+__device__ __forceinline__ float from_prev_lane(float v) {
+  float r = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+  asm volatile("" : "+v"(r));
+  return r;
+}
+__device__ __forceinline__ float from_next_lane(float v) {
+  float r = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+  asm volatile("" : "+v"(r));
+  return r;
+}
+
+// set_boundaries rule of one cell in one mode (k_rb_band): low nibble of the
+// per-cell byte, bit 4 = the cell is interior (updated by the sweeps).
+enum BandSel : uint32_t {
+  BS_KEEP = 0,
+  BS_NEG_LEFT = 1, BS_NEG_RIGHT = 2,   // AdjustRow, right wins (fluid.rs:151-164)
+  BS_NEG_UP = 3, BS_NEG_DOWN = 4,      // AdjustColumn, up wins (fluid.rs:165-177)
+  BS_LEFT = 5, BS_RIGHT = 6, BS_UP = 7, BS_DOWN = 8,  // Passive ring copies (fluid.rs:179-187)
+  BS_UPD = 16,
+};
+
+// Red-black Gauss-Seidel with set_boundaries, T full sweeps per launch, all
+// in registers.  A workgroup of NW waves owns a region of 128 columns x
+// NW*R rows: lane l holds columns 2l and 2l+1 of its wave's band of R rows
+// (x, x0 and the cell's rule byte in registers).  Horizontal neighbours come
+// from the lane's other column or the neighbouring lane (DPP), vertical ones
+// from the band's next register row; only the band's first and last rows
+// cross waves, through LDS (double-buffered, one barrier per exchange).
+// Red cells have (x + y) even; the region origin is chosen so that row i of
+// every band has its red cell in column (i & 1) of the lane.  Every full
+// sweep shrinks the valid region by 3 cells (red reads black, black reads the
+// new red, set_boundaries reads a neighbour), so the launch stores the
+// (128 - 6T) x (NW*R - 6T) interior of its region; the region reads one
+// buffer and the tile is written to the other (ping-pong).  Corners are set
+// once after the lin_solve (nothing reads them during the sweeps).
+template <int MODE, int T, int NW, int R>
+__global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict__ dp, const float* __restrict__ src,
+                                                     float* __restrict__ dst, const float* __restrict__ x0g, float a,
+                                                     float cr, int nsweeps) {
+  constexpr int H = 3 * T, W = 128, RH = NW * R;
+  constexpr int TX = W - 2 * H, TY = RH - 2 * H;
+  static_assert(TX > 0 && TY > 0 && TX % 2 == 0 && TY % 2 == 0 && R % 2 == 0, "band geometry");
+  __shared__ float2 edg[2][2][NW][64];  // [buffer][first / last row][wave][lane]
   const SceneDesc& d = *dp;
   const int S = d.S, N = d.N;
-  const int gx0 = blockIdx.x * TX - H, gy0 = blockIdx.y * TY - H;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gx0 = blockIdx.x * TX - H;  // region origin; gx0 + gy0 is even (TX, TY, R even)
+  const int gyb = blockIdx.y * TY - H;
+  const int gy0 = gyb + w * R;          // this wave's band
+  const int gxa = gx0 + 2 * lane;
   const uint8_t* __restrict__ code = d.code;
   const uint8_t* __restrict__ rowf = d.rowf;
-  for (int e = threadIdx.x; e < RW * RH; e += 256) {
-    const int ry = e / RW, rx = e - ry * RW;
-    const int gx = gx0 + rx, gy = gy0 + ry;
-    const bool in = gx >= 0 && gx < S && gy >= 0 && gy < S;
-    const int g = gy * S + gx;
-    sx[ry * P + rx] = in ? src[g] : 0.0f;
-    s0[ry * P + rx] = in ? x0g[g] : 0.0f;
-    uint8_t c = in ? code[g] : 0;
-    // Ring cells carry their Passive copy rule in the FLUID bit: a ring cell
-    // of column i / row j is rewritten iff that column / row has a NoWall cell.
-    if (in && (gy == 0 || gy == S - 1) && gx > 0 && gx < S - 1) c = (code[(gy == 0 ? 1 : S - 2) * S + gx] & C_COLF) ? 1 : 0;
-    if (in && (gx == 0 || gx == S - 1) && gy > 0 && gy < S - 1) c = rowf[gy] ? 1 : 0;
-    scode[ry * P + rx] = c;
+
+  float x[R][2], x0[R][2];
+  uint32_t cd[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int gy = gy0 + i;
+    uint32_t c2 = 0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int gx = gxa + b;
+      const bool in = gx >= 0 && gx < S && gy >= 0 && gy < S;
+      const int64_t g = (int64_t)gy * S + gx;
+      x[i][b] = in ? src[g] : 0.0f;
+      x0[i][b] = in ? x0g[g] : 0.0f;
+      uint32_t sel = 0;
+      if (in) {
+        if (gx >= 1 && gx <= N && gy >= 1 && gy <= N) {
+          const uint32_t c = code[g];
+          sel = BS_UPD;
+          if (MODE == M_ROW && (c & C_FLUID)) sel |= (c & C_RWALL) ? BS_NEG_RIGHT : (c & C_LWALL) ? BS_NEG_LEFT : 0;
+          if (MODE == M_COL && (c & C_FLUID)) sel |= (c & C_UWALL) ? BS_NEG_UP : (c & C_DWALL) ? BS_NEG_DOWN : 0;
+        } else if (MODE == M_PASSIVE) {
+          // a ring cell of column i / row j is rewritten iff that column / row has a NoWall cell
+          if (gy == 0 && gx >= 1 && gx <= N) sel = (code[S + gx] & C_COLF) ? BS_DOWN : 0;
+          else if (gy == S - 1 && gx >= 1 && gx <= N) sel = (code[(int64_t)(S - 2) * S + gx] & C_COLF) ? BS_UP : 0;
+          else if (gx == 0 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_RIGHT : 0;
+          else if (gx == S - 1 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_LEFT : 0;
+        }
+      }
+      c2 |= sel << (8 * b);
+    }
+    cd[i] = c2;
   }
-  __syncthreads();
-  constexpr int HW = (RW + 1) / 2;
+  // Wave-uniform shortcuts: a band whose cells are all interior needs no
+  // update mask, and one without set_boundaries rules skips that phase.
+  uint32_t allu = 0xffffffffu, anys = 0;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    allu &= cd[i];
+    anys |= cd[i];
+  }
+  const bool full = __builtin_amdgcn_ballot_w64((allu & (BS_UPD | (BS_UPD << 8))) != (BS_UPD | (BS_UPD << 8))) == 0;
+  const bool rules = __builtin_amdgcn_ballot_w64((anys & 0x0f0fu) != 0) != 0;
+
+  int par = 0;
   for (int k = 0; k < nsweeps; ++k) {
-    for (int color = 0; color < 2; ++color) {
-      for (int e = threadIdx.x; e < RH * HW; e += 256) {
-        const int ry = e / HW, m = e - ry * HW;
-        const int rx = ((color + gx0 + gy0 + ry) & 1) + 2 * m;
-        const int gx = gx0 + rx, gy = gy0 + ry;
-        if (rx >= 1 && rx < RW - 1 && ry >= 1 && ry < RH - 1 && gx >= 1 && gx <= N && gy >= 1 && gy <= N) {
-          const int q = ry * P + rx;
-          sx[q] = (s0[q] + a * (((sx[q + 1] + sx[q - 1]) + sx[q + P]) + sx[q - P])) * cr;  // fluid.rs:315-320
+#pragma unroll
+    for (int ph = 0; ph < 3; ++ph) {  // red, black, set_boundaries
+      // the rule bytes are loop-invariant; keeping the compilers from hoisting
+      // every per-cell lane mask out of the sweep loop (into scalar registers)
+#pragma unroll
+      for (int i = 0; i < R; ++i) asm volatile("" : "+v"(cd[i]));
+      float2 ab = make_float2(0.0f, 0.0f), be = make_float2(0.0f, 0.0f);  // rows above / below the band
+      if (ph < 2 || MODE != M_ROW) {
+        edg[par][0][w][lane] = make_float2(x[0][0], x[0][1]);
+        edg[par][1][w][lane] = make_float2(x[R - 1][0], x[R - 1][1]);
+        __syncthreads();
+        if (w > 0) ab = edg[par][1][w - 1][lane];
+        if (w < NW - 1) be = edg[par][0][w + 1][lane];
+        par ^= 1;
+      }
+      if (ph < 2) {
+        // one colour: fluid.rs:315-320 on every cell with (x + y) % 2 == ph
+        if (full) {
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            const int b = (i + ph) & 1;
+            const float left = b ? x[i][0] : from_prev_lane(x[i][1]);
+            const float right = b ? from_next_lane(x[i][0]) : x[i][1];
+            const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
+            const float down = i < R - 1 ? x[i + 1][b] : (b ? be.y : be.x);
+            x[i][b] = (x0[i][b] + a * (((right + left) + down) + up)) * cr;
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < R; ++i) {
+            const int b = (i + ph) & 1;
+            const float left = b ? x[i][0] : from_prev_lane(x[i][1]);
+            const float right = b ? from_next_lane(x[i][0]) : x[i][1];
+            const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
+            const float down = i < R - 1 ? x[i + 1][b] : (b ? be.y : be.x);
+            float nv = (x0[i][b] + a * (((right + left) + down) + up)) * cr;
+            asm volatile("" : "+v"(nv));  // a select, not a branch around the update
+            x[i][b] = ((cd[i] >> (8 * b)) & BS_UPD) ? nv : x[i][b];
+          }
+        }
+      } else if (rules) {
+        // set_boundaries (fluid.rs:141-188): writes only NoWall or ring
+        // cells and reads only walls or interior cells, so in place is safe
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const uint32_t sel = (cd[i] >> (8 * b)) & 15u;
+            float v = x[i][b];
+            if constexpr (MODE == M_ROW || MODE == M_PASSIVE) {
+              const float left = b ? x[i][0] : from_prev_lane(x[i][1]);
+              const float right = b ? from_next_lane(x[i][0]) : x[i][1];
+              if constexpr (MODE == M_ROW) {
+                v = sel == BS_NEG_RIGHT ? -right : (sel == BS_NEG_LEFT ? -left : v);
+              } else {
+                v = sel == BS_RIGHT ? right : (sel == BS_LEFT ? left : v);
+              }
+            }
+            if constexpr (MODE == M_COL || MODE == M_PASSIVE) {
+              const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
+              const float down = i < R - 1 ? x[i + 1][b] : (b ? be.y : be.x);
+              if constexpr (MODE == M_COL) {
+                v = sel == BS_NEG_UP ? -up : (sel == BS_NEG_DOWN ? -down : v);
+              } else {
+                v = sel == BS_DOWN ? down : (sel == BS_UP ? up : v);
+              }
+            }
+            x[i][b] = v;
+          }
         }
       }
-      __syncthreads();
-    }
-    // set_boundaries phase 1 (fluid.rs:141-188): writes only NoWall or ring
-    // cells, reads only walls or interior cells, so it is order-free.
-    for (int e = threadIdx.x; e < RW * RH; e += 256) {
-      const int ry = e / RW, rx = e - ry * RW;
-      const int gx = gx0 + rx, gy = gy0 + ry;
-      if (rx < 1 || rx >= RW - 1 || ry < 1 || ry >= RH - 1 || gx < 0 || gx >= S || gy < 0 || gy >= S) continue;
-      const int q = ry * P + rx;
-      const uint32_t c = scode[q];
-      if constexpr (MODE == M_ROW) {
-        if (gx >= 1 && gx <= N && gy >= 1 && gy <= N && (c & C_FLUID)) {
-          if (c & C_RWALL) sx[q] = -sx[q + 1];
-          else if (c & C_LWALL) sx[q] = -sx[q - 1];
-        }
-      } else if constexpr (MODE == M_COL) {
-        if (gx >= 1 && gx <= N && gy >= 1 && gy <= N && (c & C_FLUID)) {
-          if (c & C_UWALL) sx[q] = -sx[q - P];
-          else if (c & C_DWALL) sx[q] = -sx[q + P];
-        }
-      } else {
-        if (c & 1) {
-          if (gy == 0 && gx >= 1 && gx <= N) sx[q] = sx[q + P];
-          else if (gy == S - 1 && gx >= 1 && gx <= N) sx[q] = sx[q - P];
-          else if (gx == 0 && gy >= 1 && gy <= N) sx[q] = sx[q + 1];
-          else if (gx == S - 1 && gy >= 1 && gy <= N) sx[q] = sx[q - 1];
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // corners (fluid.rs:265-271) from the final ring values
-  if (threadIdx.x < 4 && nsweeps > 0) {
-    const int cx = (threadIdx.x & 1) ? S - 1 : 0, cy = (threadIdx.x & 2) ? S - 1 : 0;
-    const int rx = cx - gx0, ry = cy - gy0;
-    if (rx >= H && rx < H + TX && ry >= H && ry < H + TY) {
-      const int q = ry * P + rx;
-      const int hx = cx == 0 ? 1 : -1, vy = cy == 0 ? P : -P;
-      // (0,0): x(1,0)+x(0,1); (0,S-1): x(1,S-1)+x(0,S-2); (S-1,0): x(S-2,0)+x(S-1,1); (S-1,S-1): x(S-2,S-1)+x(S-1,S-2)
-      sx[q] = 0.5f * (sx[q + hx] + sx[q + vy]);
     }
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < TX * TY; e += 256) {
-    const int ty = e / TX, tx = e - ty * TX;
-    const int gx = gx0 + H + tx, gy = gy0 + H + ty;
-    if (gx < S && gy < S) dst[gy * S + gx] = sx[(ty + H) * P + tx + H];
+
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int gy = gy0 + i;
+    if (gy < gyb + H || gy >= gyb + H + TY || gy >= S) continue;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int gx = gxa + b;
+      if (gx >= gx0 + H && gx < gx0 + H + TX && gx < S) dst[(int64_t)gy * S + gx] = x[i][b];
+    }
   }
 }
 
@@ -280,37 +377,41 @@ struct RBWork {
   float* tmp = nullptr;   // ping-pong field
   int* trig = nullptr;    // per-row break column
   hipGraphExec_t graph = nullptr;
-  int tile = 0;           // tile configuration index
-  int tsweeps = 4;        // sweeps per LDS block
+  int tile = 0;           // band shape index (kBands)
+  int tsweeps = 4;        // sweeps per launch
   uint64_t key = 0;       // parameters the graph was built for
 };
 
-struct TileCfg {
-  int tx, ty;
+// Band shapes (waves per workgroup x rows per wave); the region is always 128
+// columns wide.
+struct BandCfg {
+  int nw, r;
 };
-static const TileCfg kTiles[] = {{64, 64}, {128, 32}, {32, 32}};
+static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}};
 
-template <int MODE, int T, int TX, int TY>
-static void launch_gs_t(hipStream_t st, const SceneDesc* dd, const float* src, float* dst, const float* x0, float a,
+template <int MODE, int T, int NW, int R>
+static void launch_band(hipStream_t st, const SceneDesc* dd, const float* src, float* dst, const float* x0, float a,
                         float cr, int nsw, int S) {
+  constexpr int TX = 128 - 6 * T, TY = NW * R - 6 * T;
   dim3 grid((S + TX - 1) / TX, (S + TY - 1) / TY);
-  hipLaunchKernelGGL((rb::k_rb_gs<MODE, T, TX, TY>), grid, dim3(256), 0, st, dd, src, dst, x0, a, cr, nsw);
+  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), grid, dim3(64 * NW), 0, st, dd, src, dst, x0, a, cr, nsw);
 }
 
-template <int MODE, int TX, int TY>
-static void launch_gs_tile(hipStream_t st, int T, const SceneDesc* dd, const float* src, float* dst,
-                           const float* x0, float a, float cr, int nsw, int S) {
-  if (T == 1) launch_gs_t<MODE, 1, TX, TY>(st, dd, src, dst, x0, a, cr, nsw, S);
-  else if (T == 2) launch_gs_t<MODE, 2, TX, TY>(st, dd, src, dst, x0, a, cr, nsw, S);
-  else launch_gs_t<MODE, 4, TX, TY>(st, dd, src, dst, x0, a, cr, nsw, S);
+template <int MODE, int NW, int R>
+static void launch_band_t(hipStream_t st, int T, const SceneDesc* dd, const float* src, float* dst,
+                          const float* x0, float a, float cr, int nsw, int S) {
+  if (T == 1) launch_band<MODE, 1, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
+  else if (T == 2) launch_band<MODE, 2, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
+  else if (T == 4) launch_band<MODE, 4, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
+  else launch_band<MODE, 8, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
 }
 
 template <int MODE>
 static void launch_gs_mode(hipStream_t st, int tile, int T, const SceneDesc* dd, const float* src, float* dst,
                            const float* x0, float a, float cr, int nsw, int S) {
-  if (tile == 1) launch_gs_tile<MODE, 128, 32>(st, T, dd, src, dst, x0, a, cr, nsw, S);
-  else if (tile == 2) launch_gs_tile<MODE, 32, 32>(st, T, dd, src, dst, x0, a, cr, nsw, S);
-  else launch_gs_tile<MODE, 64, 64>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+  if (tile == 1) launch_band_t<MODE, 8, 32>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+  else if (tile == 2) launch_band_t<MODE, 16, 16>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+  else launch_band_t<MODE, 8, 16>(st, T, dd, src, dst, x0, a, cr, nsw, S);
 }
 
 // lin_solve in red-black order: ceil(iters/T) tile passes, ping-ponging
@@ -334,6 +435,8 @@ static void rb_lin_solve(eqf_scene* s, RBWork* w, int mode, float* x, const floa
     const int64_t n = (int64_t)S * S;
     hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, x, a_, n);
   }
+  // corners of the last set_boundaries (fluid.rs:265-271)
+  hipLaunchKernelGGL(rb::k_corners, dim3(1), dim3(2), 0, s->stream, s->d_desc, x, (float*)nullptr);
 }
 
 static void rb_bnd(eqf_scene* s, int mode0, float* x0, int mode1, float* x1) {
@@ -395,8 +498,10 @@ void* rb_create(eqf_scene* s, std::string& err) {
   // Default tiling; the autotuner (eqf_scene_tune) may override it.
   w->tile = 0;
   w->tsweeps = 4;
-  if (const char* t = getenv("EQF_RB_TSWEEPS")) w->tsweeps = std::max(1, std::min(4, atoi(t)));
-  if (w->tsweeps == 3) w->tsweeps = 2;
+  if (const char* t = getenv("EQF_RB_TSWEEPS")) {
+    const int v = atoi(t);
+    w->tsweeps = v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1;
+  }
   if (const char* t = getenv("EQF_RB_TILE")) w->tile = std::max(0, std::min(2, atoi(t)));
   return w;
 }
@@ -413,7 +518,7 @@ void rb_destroy(void* p) {
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {
   auto* w = static_cast<RBWork*>(s->rb);
   if (!w) return EQF_EINVAL;
-  if (tile < 0 || tile > 2 || (tsweeps != 1 && tsweeps != 2 && tsweeps != 4)) return EQF_EINVAL;
+  if (tile < 0 || tile > 2 || (tsweeps != 1 && tsweeps != 2 && tsweeps != 4 && tsweeps != 8)) return EQF_EINVAL;
   if (w->tile != tile || w->tsweeps != tsweeps) {
     w->tile = tile;
     w->tsweeps = tsweeps;

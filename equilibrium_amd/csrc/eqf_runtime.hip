@@ -142,6 +142,7 @@ static void fill_desc(eqf_scene* s) {
   d.sf = (float)s->S;
   d.sfm1 = d.sf - 1.0f;
   d.has_src = s->d_src != nullptr;
+  d.prio = 0;
 }
 
 static int sync_desc(eqf_scene* s) {
@@ -470,7 +471,10 @@ extern "C" int eqf_download(eqf_scene* s, int field, float* host) {
   return download_rm(s, field_ptr(s, field), host);
 }
 
-static int waves_for(int ns) { return std::max(1, std::min(16, ns)); }
+int eqf_scene_waves(int ns, int P) {
+  const int m = std::max(1, std::min(16 / std::max(ns, 1), P / 64));  // sweep streams per strip
+  return std::max(1, std::min(16, ns * m));
+}
 
 extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
@@ -485,7 +489,7 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
     rc = rb_step(s, nframes, err);
     return rc ? fail(rc, err) : EQF_OK;
   }
-  const int W = waves_for(s->ns);
+  const int W = eqf_scene_waves(s->ns, s->P);
   while (nframes > 0) {
     const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
     hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(1), dim3(64 * W),
@@ -540,7 +544,7 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
     return (int64_t)a->N * a->N * std::max(a->desc.diff_iters, 1) > (int64_t)c->N * c->N * std::max(c->desc.diff_iters, 1);
   });
   std::vector<SceneDesc> descs(n);
-  int maxns = 1;
+  int maxns = 1, maxw = 1;
   for (int i = 0; i < n; ++i) {
     eqf_scene* s = b->scenes[i];
     int rc = sync_desc(s);
@@ -551,8 +555,10 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
     HIPCHK(hipStreamSynchronize(s->stream));
     descs[i] = s->desc;
     maxns = std::max(maxns, s->ns);
+    maxw = std::max(maxw, eqf_scene_waves(s->ns, s->P));
   }
-  b->W = waves_for(maxns);
+  for (int i = 0; i < n; ++i) descs[i].prio = eqf_batch_prio(b->scenes[i]->ns, maxns);
+  b->W = maxw;
   hipError_t e = hipStreamCreate(&b->stream);
   if (e == hipSuccess) e = hipMalloc(&b->d_descs, sizeof(SceneDesc) * n);
   if (e == hipSuccess)
@@ -570,9 +576,8 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
   return EQF_OK;
 }
 
-hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int maxns, int lds_floats, int nframes,
+hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int W, int lds_floats, int nframes,
                                hipStream_t st) {
-  const int W = waves_for(maxns);
   const size_t lds = sizeof(float) * (size_t)lds_floats;
   if (W <= 8)
     hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
@@ -582,17 +587,20 @@ hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int maxns, int l
   return hipGetLastError();
 }
 
+int eqf_batch_prio(int ns, int maxns) {
+  int p = 3;
+  for (int m = maxns; p > 0 && 2 * ns <= m; m /= 2) --p;
+  return p;
+}
+
 int eqf_internal_prepare(eqf_scene* s) { return sync_desc(s); }
 int eqf_internal_error(int code, const std::string& msg) { return fail(code, msg); }
 int eqf_internal_check_params(const eqf_params* p) { return check_params(p); }
 
 static hipError_t batch_launch(eqf_batch* b, int nframes_per_launch) {
-  int maxns = 1, lds = 0;
-  for (auto* s : b->scenes) {
-    maxns = std::max(maxns, s->ns);
-    lds = std::max(lds, cyc_lds_floats(s->ns, s->P));
-  }
-  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), maxns, lds, nframes_per_launch, b->stream);
+  int lds = 0;
+  for (auto* s : b->scenes) lds = std::max(lds, cyc_lds_floats(s->ns, s->P));
+  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), b->W, lds, nframes_per_launch, b->stream);
 }
 
 extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {

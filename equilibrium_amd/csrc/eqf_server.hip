@@ -163,9 +163,13 @@ static void server_loop(eqf_server* sv) {
     if (!act.empty()) {
       std::stable_sort(act.begin(), act.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
       int maxns = 1;
+      for (Req* r : act) maxns = std::max(maxns, r->scene->ns);
       if (act != last_list) {
         descs.resize(act.size());
-        for (size_t i = 0; i < act.size(); ++i) descs[i] = act[i]->scene->desc;
+        for (size_t i = 0; i < act.size(); ++i) {
+          descs[i] = act[i]->scene->desc;
+          descs[i].prio = eqf_batch_prio(act[i]->scene->ns, maxns);
+        }
         if ((int)act.size() > sv->desc_cap) {
           (void)hipFree(sv->d_descs);
           sv->desc_cap = std::max<int>((int)act.size(), sv->desc_cap * 2);
@@ -175,12 +179,12 @@ static void server_loop(eqf_server* sv) {
                              sv->stream);
         last_list = act;
       }
-      int lds = 0;
+      int lds = 0, waves = 1;
       for (Req* r : act) {
-        maxns = std::max(maxns, r->scene->ns);
         lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
+        waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
       }
-      hipError_t e = eqf_internal_launch(sv->d_descs, (int)act.size(), maxns, lds, 1, sv->stream);
+      hipError_t e = eqf_internal_launch(sv->d_descs, (int)act.size(), waves, lds, 1, sv->stream);
       if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->thread_error = hipGetErrorString(e);

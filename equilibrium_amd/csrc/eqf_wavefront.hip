@@ -174,16 +174,28 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
                                              gcf* __restrict__ X0, float a, float cr, int iters) {
   static_assert(16 % B == 0 && D < B, "B must divide 16 and exceed the prefetch depth");
   constexpr int L = (64 + B + D + B - 1) / B * B;  // strip lag, a multiple of B
+  constexpr int LAG = 64;                          // sweep lag between streams of a strip
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
   const int wave = uni(threadIdx.x >> 6);
   const int RO = ns * P * 64;
   gf* __restrict__ ring = X + RO;
-  const int G = (ns - 1) * L + 64 + iters * P + 1;
+  // Sweep streams: with m waves per strip, stream j of a strip runs sweeps
+  // j, j+m, j+2m, ... back to back, LAG steps behind stream j-1 (stream 0
+  // starts its next sweep P >= m*LAG steps after its last one, so every
+  // stream stays LAG behind its predecessor).  Sweep k+1 thus reads sweep
+  // k's values LAG steps after they were written, while they are still in
+  // cache, and the strip's chain is iters*P/m steps instead of iters*P.
+  int m = min(uni(blockDim.x >> 6) / ns, P / LAG);
+  m = max(1, min(m, iters));
+  const int nq0 = (iters + m - 1) / m;  // sweeps of stream 0
+  const int G = (ns - 1) * L + (m - 1) * LAG + nq0 * P + 65;
   const int Gr = (G + B - 1) / B * B;
 
-  const bool has_strip = wave < ns;
-  const int s = has_strip ? wave : 0;
+  const bool has_strip = wave < ns * m;
+  const int s = has_strip ? wave % ns : 0;  // strip
+  const int jst = has_strip ? wave / ns : 0;  // stream
+  const int nq = (iters - jst + m - 1) / m;  // sweeps of this stream
   const int j = 64 * s + lane + 1;
   const bool valid = has_strip && j <= N;
   const int rl = min(63, N - 1 - 64 * s);  // last valid lane of the strip
@@ -204,15 +216,15 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   // bottom ring is mirrored into the unused row below it, lane rl+1 of the
   // strip, and arrives through the ordinary lane shift.  Entry ring values
   // are what Passive sweeps need: they read the changing ring through xself.
-  if (has_strip && s == 0)
+  if (has_strip && jst == 0 && s == 0)
     for (int t = lane; t < P; t += 64) H[t] = t < N ? ring[t + 1] : 0.0f;
-  if (has_strip && s > 0)  // entry row 0 of this strip: the "previous sweep" of sweep 0
+  if (has_strip && jst == 0 && s > 0)  // entry row 0 of this strip: the "previous sweep" of sweep 0
     for (int t = lane; t < P; t += 64) {
       int c = t - 63;  // column - 1 of the reading lane 63 of the strip above at its slot t
       c += c < 0 ? P : 0;
       H[(ns - 1 + s) * P + t] = c < N ? X[(s * P + c) * 64] : 0.0f;
     }
-  if (has_strip && s == ns - 1)
+  if (has_strip && jst == 0 && s == ns - 1)
     for (int t = lane; t < P; t += 64) {
       int c = t - rl;
       c += c < 0 ? P : 0;
@@ -250,17 +262,18 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int sb = uni(s * P * 256);                        // strip base in X (bytes)
   const int cb = uni(s * P * 64);                        // strip base in the code array (bytes)
   const int fxb = uni(has_prev ? sb - P * 256 + 252 : 0);  // lane 63 of the strip above (bytes)
-  const int itP = uni(iters * P);
+  const int itP = uni(nq * P);  // positions of this stream
   const int o64 = uni(64 % P);
 
   float fxs[D], fx0[D], fe[D], fe2[D];  // fe / fe2: the edge value, as the up / down shift's edge operand
   uint32_t fcw[4];  // packed codes of steps 4k..4k+3 in fcw[k & 3], loaded 13-16 steps ahead
 
-  // Strip s idles for its first s*L/B blocks (every lane before its first
-  // sweep): those blocks are barriers only.  Its first working block then
-  // starts at strip-local step tau = 0, slot 0.
+  // Stream j of strip s idles for its first (s*L + j*LAG)/B blocks (every
+  // lane before its first sweep): those blocks are barriers only.  Its first
+  // working block then starts at stream-local step tau = 0, slot 0; stream
+  // position u = tau - lane covers sweep jst + m*(u div P), column u mod P.
   const int nblk = Gr / B;
-  const int head = has_strip ? s * (L / B) : nblk;
+  const int head = has_strip ? s * (L / B) + jst * (LAG / B) : nblk;
   int tau = 0;       // strip-local step of the block's first step
   int tb = 0;        // its slot (uniform)
   int tbp = P - B;   // previous block's slot
@@ -286,6 +299,9 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) fcw[k] = ldu32(rC, vl + k * 256, cb);  // slots 0..15
+    // xself of the first step: what the step before would have loaded (slot
+    // 0, lane 0's own column-1 cell, from the previous sweep when jst > 0)
+    xself = ldf(rX, vl, sb);
 #pragma nounroll
     while (blk < nblk) {
       if (tau >= 64 && tau + B <= itP) {
@@ -673,6 +689,14 @@ template <int D, int B, int MAXT>
 __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes) {
   extern __shared__ float eqf_lds[];  // cyc_lds_floats(ns, P) floats, sized by the launch
   const eqf::SceneDesc& sc = descs[blockIdx.x];
+  // The largest scenes of a batch set the launch's length: their waves win
+  // issue arbitration against co-resident smaller scenes.
+  switch (sc.prio) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: break;
+  }
   for (int fr = 0; fr < nframes; ++fr) eqf::frame_cyclic<D, B>(sc, (eqf::lf*)eqf_lds);
 }
 

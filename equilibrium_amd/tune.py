@@ -1,4 +1,4 @@
-"""Autotuner for the red-black mode: tile shape x sweeps per LDS block.
+"""Autotuner for the red-black mode: band shape x sweeps per launch.
 
 For a scene it times one frame per configuration on the GPU and keeps the
 fastest; results are cached per (size, sweeps, device name) in a JSON file
@@ -12,8 +12,9 @@ import time
 
 from . import _abi
 
-TILES = {0: (64, 64), 1: (128, 32), 2: (32, 32)}
-SWEEPS_PER_BLOCK = (1, 2, 4)
+# band shape index -> (waves per workgroup, rows per wave); regions are 128 columns wide
+TILES = {0: (8, 16), 1: (8, 32), 2: (16, 16)}
+SWEEPS_PER_BLOCK = (2, 4, 8)
 CACHE = os.environ.get("EQF_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                        "tuning_cache.json"))
 
@@ -49,7 +50,7 @@ def tune_redblack(scene, frames=1, use_cache=True):
     returns (tile, sweeps, timings).  The scene is advanced by the trial
     frames; callers that need an untouched state tune a clone."""
     p = scene.params
-    key = f"rb:{p.size}:{p.gs_iters}:{p.pressure_iters}:{_device_name()}"
+    key = f"rbband:{p.size}:{p.gs_iters}:{p.pressure_iters}:{_device_name()}"
     db = _load() if use_cache else {}
     if key in db:
         tile, tsw = db[key]["best"]

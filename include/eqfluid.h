@@ -125,9 +125,10 @@ int eqf_step(eqf_scene *s, int64_t nframes);
 /* Waits for the scene's queued work. */
 int eqf_sync(eqf_scene *s);
 
-/* Red-black mode tiling (no counterpart in the reference): tile shape index
- * (0: 64x64, 1: 128x32, 2: 32x32 cells) and Gauss-Seidel sweeps per LDS
- * block (1, 2 or 4).  Chosen by the autotuner (equilibrium_amd/tune.py). */
+/* Red-black mode tiling (no counterpart in the reference): band shape index
+ * (0: 8 waves x 16 rows, 1: 8 waves x 32 rows, 2: 16 waves x 16 rows; regions
+ * 128 columns wide) and Gauss-Seidel sweeps per launch (1, 2, 4 or 8).
+ * Chosen by the autotuner (equilibrium_amd/tune.py). */
 int eqf_scene_set_tuning(eqf_scene *s, int tile, int sweeps_per_block);
 
 /* Batches: many scenes (any sizes, wavefront mode) advanced by one fused

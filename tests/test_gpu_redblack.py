@@ -1,7 +1,7 @@
 """Red-black mode on the GPU.
 
-* Bit-for-bit equality with the red-black f32 CPU oracle for every tile shape
-  and every sweeps-per-LDS-block setting (temporal blocking must not change a
+* Bit-for-bit equality with the red-black f32 CPU oracle for every band shape
+  and every sweeps-per-launch setting (temporal blocking must not change a
   single bit: halo cells are recomputed with the same operations).
 * Accuracy against the red-black f64 oracle: norm-wise relative error of
   density and velocity and the pressure Poisson residual.
@@ -21,7 +21,7 @@ def _case(label):
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2])
-@pytest.mark.parametrize("tsw", [1, 2, 4])
+@pytest.mark.parametrize("tsw", [1, 2, 4, 8])
 def test_redblack_bit_exact_all_tilings(tile, tsw):
     N = 150
     _case(f"rb tile={tile} T={tsw} N={N}")
