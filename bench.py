@@ -207,6 +207,65 @@ def bench_redblack_4096(steps, check_frames, N=4096):
     return res
 
 
+def bench_snapshots(frames, depth=2, N=4096, tiling=None):
+    """Per-frame hand-over of the state to a consumer on the host (the
+    reference clones the Fluid into the renderer's channel after every step):
+    the N=4096 red-black scene advanced frame by frame (a) with no readback,
+    (b) with a synchronous density download per frame, (c) through a snapshot
+    ring of `depth` pinned slots carrying density and the rendered RGBA image,
+    whose copies run behind the following steps."""
+    sc = scenes.redblack_scene(N=N)
+    h = _abi.Scene(scenes.scene_params(sc, _abi.GS_REDBLACK))
+    scenes.apply_to_scene(h, sc)
+    if tiling:
+        h.set_tuning(*tiling)
+    h.step(1)
+    h.sync()
+
+    def timed(body):
+        t0 = time.perf_counter()
+        body()
+        h.sync()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    def plain():
+        for _ in range(frames):
+            h.step(1)
+
+    def sync_download():
+        for _ in range(frames):
+            h.step(1)
+            h.sync()
+            h.download(_abi.DENSITY)
+
+    snap = _abi.Snapshot(h, depth, with_rgba=True)
+    handed = []
+
+    def ring():
+        for f in range(frames):
+            if f >= depth:
+                fr, d, img = snap.acquire(copy=False)
+                handed.append(fr)
+                snap.release()
+            h.step(1)
+            snap.capture(f)
+        for _ in range(min(depth, frames)):
+            fr, d, img = snap.acquire(copy=False)
+            handed.append(fr)
+            snap.release()
+
+    ms = {"no_readback": timed(plain), "sync_download_density": timed(sync_download),
+          "snapshot_ring_density_rgba": timed(ring)}
+    snap.close()
+    h.close()
+    assert handed == list(range(frames)), handed
+    mb = (N + 2) ** 2 * 8 / 1e6
+    return {"unit": "ms/frame", "frames_timed": frames, "depth": depth,
+            "bytes_per_snapshot_mb": mb, **ms,
+            "ring_overhead_frac": ms["snapshot_ring_density_rgba"] / ms["no_readback"] - 1.0,
+            "sync_overhead_frac": ms["sync_download_density"] / ms["no_readback"] - 1.0}
+
+
 def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
     """Poisson arrivals of scene requests (N log-uniform in
     [64, n_max], frames U(50,500), sources and obstacles as in small_scene),
@@ -258,13 +317,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="headline batch only (for profiling)")
-    ap.add_argument("--legs", type=str, default="batch,single,n1024,rb4096,serve,cpu",
+    ap.add_argument("--legs", type=str, default="batch,single,n1024,rb4096,snap,serve,cpu",
                     help="comma list of legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rb-frames", type=int, default=10)
     ap.add_argument("--rb-check-frames", type=int, default=2)
+    ap.add_argument("--snap-frames", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rates", type=str, default="50,500,2000")
     args = ap.parse_args()
@@ -372,6 +432,17 @@ def main():
         # N=4096 single grid, red-black mode
         out["n4096_single_redblack"] = bench_redblack_4096(args.rb_frames, args.rb_check_frames)
         log(f"N=4096 red-black: {out['n4096_single_redblack']['value']:.4g} cell-updates/s")
+
+    if "snap" in legs:
+        # per-frame snapshots of the N=4096 red-black scene (density + RGBA)
+        rb = out.get("n4096_single_redblack")
+        tiling = None
+        if rb:
+            from equilibrium_amd import tune
+            tiling = ([k for k, v in tune.TILES.items() if list(v) == rb["band_waves_rows"]][0],
+                      rb["sweeps_per_launch"])
+        out["n4096_frame_snapshots"] = bench_snapshots(args.snap_frames, tiling=tiling)
+        log(f"N=4096 snapshots: {out['n4096_frame_snapshots']}")
 
     if "serve" in legs:
         # serving with Poisson arrivals

@@ -67,6 +67,20 @@ class ServerStats(ctypes.Structure):
 
 REQ_QUEUED, REQ_ACTIVE, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(5)
 
+
+class Colors(ctypes.Structure):
+    _fields_ = [("fluid", ctypes.c_uint8 * 4), ("world", ctypes.c_uint8 * 4), ("obstacle", ctypes.c_uint8 * 4)]
+
+
+def make_colors(fluid=None, world=None, obstacle=None):
+    """eqf_colors with the library defaults where an argument is None."""
+    c = Colors()
+    lib().eqf_colors_default(ctypes.byref(c))
+    for name, val in (("fluid", fluid), ("world", world), ("obstacle", obstacle)):
+        if val is not None:
+            getattr(c, name)[:] = [int(v) for v in val]
+    return c
+
 _lib = None
 
 
@@ -127,6 +141,14 @@ def lib():
             "eqf_server_wait_idle": (I, [P, ctypes.c_double]),
             "eqf_server_stats_get": (I, [P, ctypes.POINTER(ServerStats)]),
             "eqf_server_destroy": (I, [P]),
+            "eqf_colors_default": (None, [ctypes.POINTER(Colors)]),
+            "eqf_render_rgba": (I, [P, ctypes.POINTER(Colors), P]),
+            "eqf_snapshot_create": (I, [P, I, I, ctypes.POINTER(Colors), PP]),
+            "eqf_snapshot_capture": (I, [P, I64]),
+            "eqf_snapshot_acquire": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(ctypes.POINTER(F)),
+                                         ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))]),
+            "eqf_snapshot_release": (I, [P]),
+            "eqf_snapshot_destroy": (I, [P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -261,6 +283,58 @@ class Scene:
 
     def set_tuning(self, tile, sweeps_per_block):
         check(lib().eqf_scene_set_tuning(self.h, tile, sweeps_per_block))
+
+    def render_rgba(self, colors=None):
+        """(size*size, 4) uint8 image of the current state (eqf_render_rgba)."""
+        out = np.empty((self.size * self.size, 4), np.uint8)
+        check(lib().eqf_render_rgba(self.h, None if colors is None else ctypes.byref(colors), ptr(out)))
+        return out
+
+
+class Snapshot:
+    """Ring of per-frame host snapshots of one scene (eqf_snapshot_*)."""
+
+    def __init__(self, scene, depth=2, with_rgba=False, colors=None):
+        self.scene = scene
+        self.size = scene.size
+        self.with_rgba = bool(with_rgba)
+        h = ctypes.c_void_p()
+        check(lib().eqf_snapshot_create(scene.h, depth, 1 if with_rgba else 0,
+                                        None if colors is None else ctypes.byref(colors), ctypes.byref(h)))
+        self.h = h
+
+    def capture(self, frame):
+        check(lib().eqf_snapshot_capture(self.h, frame))
+
+    def acquire(self, copy=True):
+        """(frame, density, rgba or None) of the oldest capture.  With
+        copy=False the arrays view the pinned slot and stay valid only until
+        release()."""
+        fr = ctypes.c_int64()
+        d = ctypes.POINTER(ctypes.c_float)()
+        c = ctypes.POINTER(ctypes.c_uint8)()
+        check(lib().eqf_snapshot_acquire(self.h, ctypes.byref(fr), ctypes.byref(d), ctypes.byref(c)))
+        n = self.size * self.size
+        dens = np.ctypeslib.as_array(d, shape=(n,))
+        rgba = np.ctypeslib.as_array(c, shape=(n, 4)) if self.with_rgba else None
+        if copy:
+            dens = dens.copy()
+            rgba = None if rgba is None else rgba.copy()
+        return fr.value, dens, rgba
+
+    def release(self):
+        check(lib().eqf_snapshot_release(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().eqf_snapshot_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -12,7 +12,8 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
-SOURCES = ["eqf_runtime.hip", "eqf_wavefront.hip", "eqf_redblack.hip", "eqf_server.hip", "eqf_setup.hip"]
+SOURCES = ["eqf_runtime.hip", "eqf_wavefront.hip", "eqf_redblack.hip", "eqf_server.hip", "eqf_setup.hip",
+           "eqf_render.hip"]
 OUT = os.path.join(PKG_DIR, "libeqfluid.so")
 ARCH = os.environ.get("EQF_OFFLOAD_ARCH", "gfx950")
 

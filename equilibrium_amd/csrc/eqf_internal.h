@@ -60,6 +60,8 @@ int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t str
 int eqf_dev_init_fields(eqf_scene* s, int times);
 int eqf_dev_upload(eqf_scene* s, float* dst, float* dst2, const float* host, int add);
 int eqf_dev_download(eqf_scene* s, const float* src, float* host);
+// src field (scene layout) -> row-major device array, queued on the scene's stream.
+int eqf_dev_to_rm_async(eqf_scene* s, const float* src, float* dst);
 int eqf_dev_rebuild_bnd(eqf_scene* s);
 void eqf_dev_trim(eqf_scene* s);
 

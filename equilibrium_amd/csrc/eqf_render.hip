@@ -1,0 +1,227 @@
+// eqf_render.hip -- the renderer's side of the path, on the device:
+//  * eqf_render_rgba: RenderingListener::render_image's pixel rule
+//    (renderer_helpers.rs:143-165) from the scene's density and mask;
+//  * snapshot rings: CurrentSimulation::simulate hands the renderer a clone
+//    of the Fluid after every step (renderer_helpers.rs:59-65).  A capture
+//    converts density (and the RGBA image) on the scene's stream behind the
+//    queued steps, then a copy stream moves it to pinned host memory, so the
+//    transfer overlaps the following steps instead of stalling them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "eqf_internal.h"
+
+using namespace eqf;
+
+namespace {
+
+// Rust `f as u8` (saturating: NaN -> 0, below 0 -> 0, above 255 -> 255,
+// otherwise truncation toward zero).
+__device__ __forceinline__ uint32_t rust_u8(float f) {
+  if (!(f > 0.0f)) return 0;
+  if (f >= 255.0f) return 255;
+  return (uint32_t)f;
+}
+
+// One pixel per thread: rgba[e] for cell e = x + y*S of the row-major
+// density `d` and mask `m` (renderer_helpers.rs:143-165).
+__global__ void k_render(int64_t n, const float* __restrict__ d, const uint8_t* __restrict__ m, uint32_t fluid,
+                         uint32_t world, uint32_t obstacle, uint32_t* __restrict__ rgba) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t cell = m[e];
+  const float dens = d[e];
+  uint32_t px;
+  if (cell == EQF_DEFAULT_WALL) {
+    px = obstacle;
+  } else if (dens != 0.0f && cell == EQF_NO_WALL) {
+    // [(density * fluid.r as f32) as u8, fluid.g, density as u8, 1]
+    px = rust_u8(dens * (float)(fluid & 0xffu)) | (fluid & 0xff00u) | (rust_u8(dens) << 16) | (1u << 24);
+  } else {
+    px = world;
+  }
+  rgba[e] = px;
+}
+
+uint32_t pack(const uint8_t c[4]) {
+  return (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// Density of the scene as a row-major device array and its mask on the
+// device (rebuilt if the mask changed since the boundary data was built).
+static int render_inputs(eqf_scene* s, float* d_dens_rm) {
+  int rc = eqf_internal_prepare(s);  // boundary data, incl. the device mask
+  if (rc) return rc;
+  if (!s->d_mask) {
+    rc = eqf_dev_rebuild_bnd(s);
+    if (rc) return rc;
+  }
+  return eqf_dev_to_rm_async(s, s->d_fld + (int64_t)F_D * s->fstride, d_dens_rm);
+}
+
+static int launch_render(eqf_scene* s, const float* d_dens_rm, const eqf_colors* c, uint8_t* d_rgba) {
+  const int64_t n = (int64_t)s->S * s->S;
+  hipLaunchKernelGGL(k_render, dim3(blocks_for(n)), dim3(256), 0, s->stream, n, d_dens_rm, s->d_mask,
+                     pack(c->fluid), pack(c->world), pack(c->obstacle), (uint32_t*)d_rgba);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? EQF_OK : eqf_internal_error(EQF_EDEVICE, std::string("render: ") + hipGetErrorString(e));
+}
+
+extern "C" void eqf_colors_default(eqf_colors* c) {
+  if (!c) return;
+  // configs.rs:56-57 (fluid_color, world_color); RenderingListener's obstacles_color
+  // is egui's Color32::RED (renderer_helpers.rs:97-103)
+  const uint8_t fl[4] = {208, 88, 157, 220}, wo[4] = {94, 146, 162, 128}, ob[4] = {255, 0, 0, 255};
+  std::copy(fl, fl + 4, c->fluid);
+  std::copy(wo, wo + 4, c->world);
+  std::copy(ob, ob + 4, c->obstacle);
+}
+
+extern "C" int eqf_render_rgba(eqf_scene* s, const eqf_colors* c, uint8_t* rgba_host) {
+  if (!s || !rgba_host) return eqf_internal_error(EQF_EINVAL, "bad arguments");
+  if (s->in_batch) return eqf_internal_error(EQF_EBUSY, "scene is part of a batch");
+  eqf_colors def;
+  if (!c) {
+    eqf_colors_default(&def);
+    c = &def;
+  }
+  hipError_t e = hipSetDevice(s->ctx->device);
+  const size_t n = (size_t)s->S * s->S;
+  float* d_dens = nullptr;
+  uint8_t* d_rgba = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&d_dens, sizeof(float) * n + 4 * n);
+  if (e != hipSuccess) return eqf_internal_error(EQF_ENOMEM, std::string("render buffer: ") + hipGetErrorString(e));
+  d_rgba = (uint8_t*)(d_dens + n);
+  int rc = render_inputs(s, d_dens);
+  if (!rc) rc = launch_render(s, d_dens, c, d_rgba);
+  if (!rc) {
+    e = hipMemcpyAsync(rgba_host, d_rgba, 4 * n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = eqf_internal_error(EQF_EDEVICE, std::string("render copy: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(d_dens);
+  return rc;
+}
+
+// ---------------------------------------------------------------- snapshots
+struct eqf_snapshot {
+  eqf_scene* scene = nullptr;
+  int device = 0;
+  int depth = 0, with_rgba = 0;
+  eqf_colors colors{};
+  hipStream_t copy = nullptr;
+  struct Slot {
+    float* d_dens = nullptr;  // device staging (row-major density, then RGBA)
+    uint8_t* d_rgba = nullptr;
+    float* h_dens = nullptr;  // pinned host copy
+    uint8_t* h_rgba = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    int64_t frame = 0;
+  };
+  std::vector<Slot> slots;
+  int head = 0, count = 0;  // oldest capture, captures held
+};
+
+static void snapshot_free(eqf_snapshot* sn) {
+  for (auto& sl : sn->slots) {
+    (void)hipFree(sl.d_dens);
+    (void)hipHostFree(sl.h_dens);
+    if (sl.ready) (void)hipEventDestroy(sl.ready);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  if (sn->copy) (void)hipStreamDestroy(sn->copy);
+  delete sn;
+}
+
+extern "C" int eqf_snapshot_create(eqf_scene* s, int depth, int with_rgba, const eqf_colors* c, eqf_snapshot** out) {
+  if (!s || !out || depth < 1 || depth > 8) return eqf_internal_error(EQF_EINVAL, "bad arguments (depth 1..8)");
+  if (s->in_batch) return eqf_internal_error(EQF_EBUSY, "scene is part of a batch");
+  hipError_t e = hipSetDevice(s->ctx->device);
+  auto* sn = new eqf_snapshot;
+  sn->scene = s;
+  sn->device = s->ctx->device;
+  sn->depth = depth;
+  sn->with_rgba = with_rgba ? 1 : 0;
+  if (c) sn->colors = *c;
+  else eqf_colors_default(&sn->colors);
+  const size_t n = (size_t)s->S * s->S;
+  const size_t bytes = sizeof(float) * n + (sn->with_rgba ? 4 * n : 0);
+  if (e == hipSuccess) e = hipStreamCreate(&sn->copy);
+  sn->slots.resize(depth);
+  for (auto& sl : sn->slots) {
+    if (e == hipSuccess) e = hipMalloc(&sl.d_dens, bytes);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_dens, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess && sn->with_rgba) {
+      sl.d_rgba = (uint8_t*)(sl.d_dens + n);
+      sl.h_rgba = (uint8_t*)(sl.h_dens + n);
+    }
+  }
+  if (e != hipSuccess) {
+    snapshot_free(sn);
+    return eqf_internal_error(EQF_ENOMEM, std::string("snapshot ring: ") + hipGetErrorString(e));
+  }
+  *out = sn;
+  return EQF_OK;
+}
+
+extern "C" int eqf_snapshot_capture(eqf_snapshot* sn, int64_t frame) {
+  if (!sn) return eqf_internal_error(EQF_EINVAL, "snapshot is NULL");
+  if (sn->count == sn->depth) return eqf_internal_error(EQF_EBUSY, "every snapshot slot holds an unreleased capture");
+  eqf_scene* s = sn->scene;
+  if (s->in_batch) return eqf_internal_error(EQF_EBUSY, "scene is part of a batch");
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, hipGetErrorString(e));
+  auto& sl = sn->slots[(sn->head + sn->count) % sn->depth];
+  const size_t n = (size_t)s->S * s->S;
+  int rc = render_inputs(s, sl.d_dens);
+  if (!rc && sn->with_rgba) rc = launch_render(s, sl.d_dens, &sn->colors, sl.d_rgba);
+  if (rc) return rc;
+  const size_t bytes = sizeof(float) * n + (sn->with_rgba ? 4 * n : 0);
+  e = hipEventRecord(sl.ready, s->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(sn->copy, sl.ready, 0);
+  if (e == hipSuccess) e = hipMemcpyAsync(sl.h_dens, sl.d_dens, bytes, hipMemcpyDeviceToHost, sn->copy);
+  if (e == hipSuccess) e = hipEventRecord(sl.done, sn->copy);
+  if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, std::string("snapshot: ") + hipGetErrorString(e));
+  sl.frame = frame;
+  ++sn->count;
+  return EQF_OK;
+}
+
+extern "C" int eqf_snapshot_acquire(eqf_snapshot* sn, int64_t* frame, const float** density, const uint8_t** rgba) {
+  if (!sn) return eqf_internal_error(EQF_EINVAL, "snapshot is NULL");
+  if (sn->count == 0) return eqf_internal_error(EQF_EINVAL, "no capture to acquire");
+  auto& sl = sn->slots[sn->head];
+  hipError_t e = hipEventSynchronize(sl.done);
+  if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, std::string("snapshot wait: ") + hipGetErrorString(e));
+  if (frame) *frame = sl.frame;
+  if (density) *density = sl.h_dens;
+  if (rgba) *rgba = sn->with_rgba ? sl.h_rgba : nullptr;
+  return EQF_OK;
+}
+
+extern "C" int eqf_snapshot_release(eqf_snapshot* sn) {
+  if (!sn) return eqf_internal_error(EQF_EINVAL, "snapshot is NULL");
+  if (sn->count == 0) return eqf_internal_error(EQF_EINVAL, "no capture to release");
+  sn->head = (sn->head + 1) % sn->depth;
+  --sn->count;
+  return EQF_OK;
+}
+
+extern "C" int eqf_snapshot_destroy(eqf_snapshot* sn) {
+  if (!sn) return EQF_OK;
+  // The copy stream's last copy waited for every capture's device work, so
+  // draining it leaves no work on the slots (the scene itself is not touched).
+  (void)hipSetDevice(sn->device);
+  (void)hipStreamSynchronize(sn->copy);
+  snapshot_free(sn);
+  return EQF_OK;
+}

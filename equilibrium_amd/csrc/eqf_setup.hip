@@ -188,6 +188,18 @@ int eqf_dev_upload(eqf_scene* s, float* dst, float* dst2, const float* host, int
   return e == hipSuccess ? EQF_OK : eqf_internal_error(EQF_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
 }
 
+int eqf_dev_to_rm_async(eqf_scene* s, const float* src, float* dst) {
+  const size_t n = (size_t)s->S * s->S;
+  hipError_t e;
+  if (s->layout == L_ROWMAJOR) {
+    e = hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, s->stream);
+  } else {
+    hipLaunchKernelGGL(k_to_rm, dim3(blocks_for((int64_t)n)), dim3(256), 0, s->stream, geo_of(s), src, dst);
+    e = hipGetLastError();
+  }
+  return e == hipSuccess ? EQF_OK : eqf_internal_error(EQF_EDEVICE, std::string("layout copy: ") + hipGetErrorString(e));
+}
+
 int eqf_dev_download(eqf_scene* s, const float* src, float* host) {
   const size_t n = (size_t)s->S * s->S;
   hipError_t e;

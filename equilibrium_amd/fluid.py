@@ -12,7 +12,11 @@ Names and argument meanings follow the Rust items they mirror:
 * ``Fluid``             -- fluid.rs:51-620; pub fields become properties that
   read the device-resident state, ``step`` runs on the GPU.
 * ``CurrentSimulation`` -- renderer_helpers.rs:29-81 frame loop (without the
-  mpsc channel: ``simulate`` yields each frame's state or calls a callback).
+  mpsc channel: ``simulate`` calls a callback per frame; ``simulate_snapshots``
+  yields per-frame host snapshots copied behind the following steps).
+* ``RenderingListener`` -- renderer_helpers.rs:87-165: ``render_image`` builds
+  the RGBA image on the device and returns it as a (size, size, 4) array
+  instead of writing a PNG file.
 """
 import enum
 import math
@@ -208,6 +212,23 @@ class Fluid:
         self.add_velocity(int(cx), int(cy), float(rx * np.float32(2.0)), float(ry * np.float32(2.0)))
 
 
+class RenderingListener:
+    """renderer_helpers.rs:87-165 (obstacles_color defaults to egui's
+    Color32::RED)."""
+
+    def __init__(self, obstacles_color=(255, 0, 0, 255)):
+        self.obstacles_color = tuple(obstacles_color)
+
+    def colors(self, fluid):
+        fc = fluid.fluid_configs
+        return _abi.make_colors(fc.fluid_color, fc.world_color, self.obstacles_color)
+
+    def render_image(self, fluid):
+        """Pixel (x, y) at [y, x] of a (size, size, 4) uint8 array."""
+        size = fluid.simulation_configs.size
+        return fluid.scene.render_rgba(self.colors(fluid)).reshape(size, size, 4)
+
+
 class CurrentSimulation:
     """renderer_helpers.rs:29-81: obstacles are marked once, then each frame
     runs (noise,) step and hands the state to the consumer."""
@@ -229,3 +250,34 @@ class CurrentSimulation:
             self.fluid.step()
             if on_frame is not None:
                 on_frame(i, self.fluid)
+
+    def simulate_snapshots(self, noise=None, depth=2, listener=None):
+        """Like ``simulate``, yielding (frame, density, rgba) per frame; the
+        copy of frame i to the host runs behind the steps of frames i+1..,
+        up to `depth` frames ahead.  rgba is the (size, size, 4) image of
+        ``listener`` (None for density only)."""
+        self.mark_fluid_obstacles()
+        f = self.fluid
+        f._sync_params()
+        size = f.simulation_configs.size
+        colors = None if listener is None else listener.colors(f)
+        snap = _abi.Snapshot(f.scene, depth, listener is not None, colors)
+        try:
+            frames = f.simulation_configs.frames
+            for i in range(frames):
+                if i >= depth:  # every slot holds a capture: hand the oldest over
+                    yield self._take(snap, size)
+                if f.fluid_configs.has_perlin_noise and noise is not None:
+                    f.add_noise(*noise(i))
+                f.scene.step(1)
+                snap.capture(i)
+            for _ in range(min(depth, frames)):
+                yield self._take(snap, size)
+        finally:
+            snap.close()
+
+    @staticmethod
+    def _take(snap, size):
+        fr, dens, rgba = snap.acquire()
+        snap.release()
+        return fr, dens, None if rgba is None else rgba.reshape(size, size, 4)

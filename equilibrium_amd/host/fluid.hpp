@@ -5,6 +5,8 @@
 // std::runtime_error with the reference's message.  Header-only; link with
 // -leqfluid.
 #pragma once
+#include <algorithm>
+#include <array>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -27,11 +29,15 @@ struct SimulationConfigs {
   static SimulationConfigs new_(float delta_t, int64_t frames, uint32_t size) { return {delta_t, frames, size}; }
 };
 
-// configs.rs:37-60 (colours omitted: they only feed the renderer)
+using Rgba = std::array<uint8_t, 4>;  // egui::Color32 as its four bytes
+
+// configs.rs:37-60
 struct FluidConfigs {
   float diffusion = 0.0f;
   float viscousity = 0.001f;
   bool has_perlin_noise = true;
+  Rgba fluid_color{208, 88, 157, 220};
+  Rgba world_color{94, 146, 162, 128};
 };
 
 using Point = std::pair<int64_t, int64_t>;  // line_drawing::Point<i64>
@@ -167,6 +173,60 @@ class Fluid {
   }
 };
 
+// renderer_helpers.rs:87-165.  render_image returns the RGBA8 pixels
+// (x + y*size, four bytes each) computed on the GPU instead of a PNG file.
+struct RenderingListener {
+  Rgba obstacles_color{255, 0, 0, 255};  // egui::Color32::RED
+
+  eqf_colors colors(const Fluid& f) const {
+    eqf_colors c;
+    for (int k = 0; k < 4; ++k) {
+      c.fluid[k] = f.fluid_configs.fluid_color[k];
+      c.world[k] = f.fluid_configs.world_color[k];
+      c.obstacle[k] = obstacles_color[k];
+    }
+    return c;
+  }
+  std::vector<uint8_t> render_image(const Fluid& f) const {
+    const eqf_colors c = colors(f);
+    std::vector<uint8_t> px((size_t)4 * f.simulation_configs.size * f.simulation_configs.size);
+    detail::check(eqf_render_rgba(f.handle(), &c, px.data()));
+    return px;
+  }
+};
+
+// One frame handed to the consumer: density (row-major) and, when asked
+// for, the rendered RGBA8 image.  Both point into a pinned ring slot that
+// stays valid until the FrameSnapshots object moves on.
+struct FrameView {
+  int64_t frame;
+  const float* density;
+  const uint8_t* rgba;
+};
+
+// Ring of per-frame host copies of one Fluid (eqf_snapshot_*).
+class FrameSnapshots {
+ public:
+  FrameSnapshots(Fluid& f, int depth, const eqf_colors* colors) {
+    eqf_snapshot* h = nullptr;
+    detail::check(eqf_snapshot_create(f.handle(), depth, colors != nullptr, colors, &h));
+    h_.reset(h);
+  }
+  void capture(int64_t frame) { detail::check(eqf_snapshot_capture(h_.get(), frame)); }
+  FrameView acquire() {
+    FrameView v{};
+    detail::check(eqf_snapshot_acquire(h_.get(), &v.frame, &v.density, &v.rgba));
+    return v;
+  }
+  void release() { detail::check(eqf_snapshot_release(h_.get())); }
+
+ private:
+  struct Del {
+    void operator()(eqf_snapshot* s) const { eqf_snapshot_destroy(s); }
+  };
+  std::unique_ptr<eqf_snapshot, Del> h_;
+};
+
 // renderer_helpers.rs:29-81 without the mpsc channel: on_frame(i, fluid)
 // receives each frame's state.
 struct CurrentSimulation {
@@ -183,6 +243,27 @@ struct CurrentSimulation {
       fluid.step();
       on_frame(i, fluid);
     }
+  }
+  // The same frames, each handed over as a FrameView whose copy to the host
+  // ran behind the following steps (up to `depth` frames in flight);
+  // listener != nullptr adds its rendered image.
+  template <class F>
+  void simulate_snapshots(F&& on_frame, int depth = 2, const RenderingListener* listener = nullptr) {
+    mark_fluid_obstacles();
+    eqf_colors c{};
+    if (listener) c = listener->colors(fluid);
+    FrameSnapshots snaps(fluid, depth, listener ? &c : nullptr);
+    const int64_t frames = fluid.simulation_configs.frames;
+    auto hand_over = [&] {
+      on_frame(snaps.acquire());
+      snaps.release();
+    };
+    for (int64_t i = 0; i < frames; ++i) {
+      if (i >= depth) hand_over();
+      fluid.step();
+      snaps.capture(i);
+    }
+    for (int64_t i = 0; i < std::min<int64_t>(depth, frames); ++i) hand_over();
   }
 };
 

@@ -202,6 +202,42 @@ int eqf_server_wait_idle(eqf_server *sv, double timeout_ms);
 int eqf_server_stats_get(eqf_server *sv, eqf_server_stats *st);
 int eqf_server_destroy(eqf_server *sv);
 
+/* ---- the renderer's side: RGBA image and per-frame snapshots ---------- */
+
+/* Renderer colours as RGBA bytes. */
+typedef struct eqf_colors {
+  uint8_t fluid[4];    /* FluidConfigs::fluid_color (configs.rs:56) */
+  uint8_t world[4];    /* FluidConfigs::world_color (configs.rs:57) */
+  uint8_t obstacle[4]; /* RenderingListener::obstacles_color (renderer_helpers.rs:101) */
+} eqf_colors;
+/* The reference defaults: (208,88,157,220), (94,146,162,128), (255,0,0,255). */
+void eqf_colors_default(eqf_colors *c);
+
+/* RenderingListener::render_image's pixels (renderer_helpers.rs:143-165),
+ * computed on the device from the scene's density and mask: size*size RGBA8
+ * pixels, row-major (x + y*size), into `rgba` (4*size*size bytes).  NULL
+ * colours = eqf_colors_default.  Synchronous. */
+int eqf_render_rgba(eqf_scene *s, const eqf_colors *c, uint8_t *rgba);
+
+/* Per-frame snapshots: CurrentSimulation::simulate sends the renderer a
+ * clone of the Fluid after every step (renderer_helpers.rs:59-65).  A ring of
+ * `depth` (1..8) pinned host slots; a capture is queued behind the scene's
+ * steps (density, row-major f32, and, with `with_rgba`, the RGBA image) and
+ * its copy to host runs on a separate stream, overlapping later steps. */
+typedef struct eqf_snapshot eqf_snapshot;
+int eqf_snapshot_create(eqf_scene *s, int depth, int with_rgba, const eqf_colors *c, eqf_snapshot **out);
+/* Queues a capture of the scene's state after its queued steps, tagged with
+ * `frame`.  EQF_EBUSY when every slot holds an unreleased capture. */
+int eqf_snapshot_capture(eqf_snapshot *sn, int64_t frame);
+/* Waits for the oldest capture; the pointers (size*size floats, 4*size*size
+ * bytes or NULL) stay valid until eqf_snapshot_release. */
+int eqf_snapshot_acquire(eqf_snapshot *sn, int64_t *frame, const float **density, const uint8_t **rgba);
+/* Frees the oldest capture's slot. */
+int eqf_snapshot_release(eqf_snapshot *sn);
+/* Waits for outstanding copies and frees the ring; call before destroying
+ * the scene.  A ring and its scene are used from one thread at a time. */
+int eqf_snapshot_destroy(eqf_snapshot *sn);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,7 @@ def lib():
         L.ora_step_par_f64.argtypes = [P] * 6 + [I64, ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, I64, P]
         L.ora_poisson_residual_f64.argtypes = [P, P, I64, P, P]
         L.ora_poisson_residual_f64.restype = ctypes.c_double
+        L.ora_render_rgba.argtypes = [P, P, I64, P, P, P, P]
         _lib = L
     return _lib
 
@@ -174,3 +175,19 @@ def poisson_residual(p, div, size):
                                         _p(np.ascontiguousarray(div, np.float64)), size,
                                         ctypes.byref(rr), ctypes.byref(rd))
     return mx, rr.value, rd.value
+
+
+FLUID_COLOR = (208, 88, 157, 220)     # configs.rs:56
+WORLD_COLOR = (94, 146, 162, 128)     # configs.rs:57
+OBSTACLE_COLOR = (255, 0, 0, 255)     # egui Color32::RED (renderer_helpers.rs:101)
+
+
+def render_rgba(density, mask, size, fluid=FLUID_COLOR, world=WORLD_COLOR, obstacle=OBSTACLE_COLOR):
+    """renderer_helpers.rs:143-165: (size*size, 4) uint8 RGBA image."""
+    d = np.ascontiguousarray(density, np.float32).ravel()
+    m = np.ascontiguousarray(mask, np.uint8).ravel()
+    assert d.size == size * size and m.size == size * size
+    out = np.empty((size * size, 4), np.uint8)
+    cols = [np.ascontiguousarray(np.array(c, np.uint8)) for c in (fluid, world, obstacle)]
+    lib().ora_render_rgba(_p(d), _p(m), size, *(_p(c) for c in cols), _p(out))
+    return out

@@ -25,6 +25,8 @@
  *   ora_rect_valid   obstacle.rs:134-147
  *   ora_step_par_f64 parallel (OpenMP) red-black f64 step, identical results
  *                    to ora_step_f64(order=ORA_RB), for large grids.
+ *   ora_render_rgba  renderer_helpers.rs:143-165 (RenderingListener::render_image
+ *                    pixel rule, `f32 as u8` with Rust's saturating semantics)
  *
  * Parity pinning: see tests/test_oracle.py (closed forms, brute force,
  * reference known answers).  add_noise (fluid.rs:575-599) is not restated:
@@ -294,4 +296,35 @@ double ora_poisson_residual_f64(const double *p, const double *div, int64_t S, d
     if (rms_r) *rms_r = sqrt(s2 / n);
     if (rms_div) *rms_div = sqrt(d2 / n);
     return mx;
+}
+
+/* Rust `f as u8` for an f32 (saturating since Rust 1.45: NaN -> 0, values
+ * below 0 -> 0, above 255 -> 255, otherwise truncation toward zero). */
+static uint8_t rust_f32_as_u8(float f) {
+  if (isnan(f) || f <= 0.0f) return 0;
+  if (f >= 255.0f) return 255;
+  return (uint8_t)(int)truncf(f);
+}
+
+/* renderer_helpers.rs:143-165: pixel (x, y) of the size x size image from
+ * density[idx(x, y)] and cells_type[idx(x, y)]; rgba holds 4 bytes per pixel,
+ * row-major (x + y*size), as image::ImageBuffer lays them out. */
+void ora_render_rgba(const float *density, const uint8_t *mask, int64_t size, const uint8_t fluid[4],
+                     const uint8_t world[4], const uint8_t obstacle[4], uint8_t *rgba) {
+  for (int64_t y = 0; y < size; ++y)
+    for (int64_t x = 0; x < size; ++x) {
+      const int64_t e = (int64_t)IDX(x, y, size);
+      uint8_t *px = rgba + 4 * (x + y * size);
+      const float d = density[e];
+      if (mask[e] == ORA_WALL) {
+        memcpy(px, obstacle, 4);
+      } else if (d != 0.0f && mask[e] == ORA_FLUID) {
+        px[0] = rust_f32_as_u8(d * (float)fluid[0]);
+        px[1] = fluid[1];
+        px[2] = rust_f32_as_u8(d);
+        px[3] = 1;
+      } else {
+        memcpy(px, world, 4);
+      }
+    }
 }

@@ -16,6 +16,7 @@ void ora_init_f32(float*, float*, float*, float*, float*, float*, uint8_t*, uint
 void ora_fill_rect(uint8_t*, uint32_t, int64_t, int64_t, int64_t, int64_t);
 void ora_run_f32(float*, float*, float*, float*, float*, float*, int64_t, float, float, float, int64_t, int64_t,
                  const uint8_t*, int, const float*, const float*, const float*, int64_t);
+void ora_render_rgba(const float*, const uint8_t*, int64_t, const uint8_t*, const uint8_t*, const uint8_t*, uint8_t*);
 }
 
 static int failures = 0;
@@ -85,6 +86,41 @@ static void default_simulation_matches_oracle() {
   CHECK(std::memcmp(got[2].data(), V.data(), V.size() * 4) == 0);
 }
 
+// The same simulation through simulate_snapshots with a RenderingListener:
+// every frame's snapshot (density and RGBA image) == the oracle's state after
+// that frame and its rendering, bit for bit, in frame order.
+static void snapshots_match_oracle() {
+  CurrentSimulation sim;
+  sim.fluid.simulation_configs.frames = 4;
+  RenderingListener listener;
+  const uint32_t S = 128;
+  std::vector<int64_t> frames;
+  std::vector<std::vector<float>> dens;
+  std::vector<std::vector<uint8_t>> imgs;
+  sim.simulate_snapshots(
+      [&](const FrameView& v) {
+        frames.push_back(v.frame);
+        dens.emplace_back(v.density, v.density + S * S);
+        imgs.emplace_back(v.rgba, v.rgba + 4 * S * S);
+      },
+      2, &listener);
+  CHECK(frames.size() == 4);
+  std::vector<float> D(S * S), Sc(S * S), U(S * S), V(S * S), U0(S * S), V0(S * S);
+  std::vector<uint8_t> m(S * S), img(4 * S * S);
+  ora_init_f32(D.data(), Sc.data(), U.data(), V.data(), U0.data(), V0.data(), m.data(), S, 2);
+  ora_fill_rect(m.data(), S, 80, 80, 110, 110);
+  const FluidConfigs fc;
+  for (size_t i = 0; i < frames.size(); ++i) {
+    CHECK(frames[i] == (int64_t)i);
+    ora_run_f32(D.data(), Sc.data(), U.data(), V.data(), U0.data(), V0.data(), S, 0.02f, 0.0f, 0.001f, 4, 4,
+                m.data(), 0, nullptr, nullptr, nullptr, 1);
+    ora_render_rgba(D.data(), m.data(), S, fc.fluid_color.data(), fc.world_color.data(),
+                    listener.obstacles_color.data(), img.data());
+    CHECK(std::memcmp(dens[i].data(), D.data(), D.size() * 4) == 0);
+    CHECK(std::memcmp(imgs[i].data(), img.data(), img.size()) == 0);
+  }
+}
+
 int main(int argc, char** argv) {
   const bool gpu = !(argc > 1 && std::strcmp(argv[1], "--cpu") == 0);
   to_coordinate();
@@ -92,6 +128,7 @@ int main(int argc, char** argv) {
   if (gpu) {
     mark_fluid_obstacles_default_current_simulation();
     default_simulation_matches_oracle();
+    snapshots_match_oracle();
   }
   std::printf("%s: %d failure(s)\n", gpu ? "cpu+gpu" : "cpu", failures);
   return failures ? 1 : 0;

@@ -458,3 +458,74 @@ def test_lexicographic_and_red_black_steps_differ():
     a.run(0.1, 1e-4, 1e-3, 5, 1, order=LEX, src_d=sc["src_d"])
     b.run(0.1, 1e-4, 1e-3, 5, 1, order=RB, src_d=sc["src_d"])
     assert not np.array_equal(a.f["density"], b.f["density"])
+
+
+# ---------------------------------------------------------------- rendering
+# Known answers for renderer_helpers.rs:143-165, each worked out by hand from
+# the rule and Rust's documented `f32 as u8` (saturating: NaN -> 0, negative
+# -> 0, >= 255 -> 255, else truncation toward zero).
+_F, _W, _O = oracle.FLUID_COLOR, oracle.WORLD_COLOR, oracle.OBSTACLE_COLOR
+RENDER_CASES = [
+    # (density, cell, expected RGBA)
+    (5.0, WALL, _O),                      # a wall shows the obstacle colour whatever its density
+    (0.0, WALL, _O),
+    (0.0, FLUID, _W),                     # no density: world colour
+    (-0.0, FLUID, _W),                    # -0.0 == 0.0
+    (0.5, FLUID, (104, 88, 0, 1)),        # 0.5*208 = 104.0; 0.5 -> 0
+    (1.0, FLUID, (208, 88, 1, 1)),
+    (0.999, FLUID, (207, 88, 0, 1)),      # 207.79.. -> 207
+    (1.3, FLUID, (255, 88, 1, 1)),        # 270.4 saturates
+    (254.99, FLUID, (255, 88, 254, 1)),
+    (300.0, FLUID, (255, 88, 255, 1)),
+    (1e-30, FLUID, (0, 88, 0, 1)),        # non-zero: fluid branch, both channels 0
+    (-3.0, FLUID, (0, 88, 0, 1)),         # negative saturates to 0
+    (float("nan"), FLUID, (0, 88, 0, 1)),  # NaN != 0, NaN as u8 = 0
+    (float("inf"), FLUID, (255, 88, 255, 1)),
+    (float("-inf"), FLUID, (0, 88, 0, 1)),
+    (1.2259615, FLUID, (254, 88, 1, 1)),  # f32(1.2259615)*208 = 254.99999 -> 254
+]
+
+
+def test_render_known_answers():
+    n = len(RENDER_CASES)
+    size = 4
+    assert n <= size * size
+    d = np.zeros(size * size, np.float32)
+    m = np.full(size * size, FLUID, np.uint8)
+    for e, (v, cell, _) in enumerate(RENDER_CASES):
+        d[e], m[e] = v, cell
+    img = oracle.render_rgba(d, m, size)
+    for e, (v, cell, want) in enumerate(RENDER_CASES):
+        assert tuple(img[e]) == tuple(want), (v, cell, tuple(img[e]), want)
+
+
+def test_render_float_product_rounding_boundary():
+    """The red channel is the f32 product rounded once, then truncated: at the
+    f32 just below 255/208 the product stays below 255, at 255/208 itself it
+    reaches 255; the blue channel (density as u8) is 1 for both."""
+    hi = np.float32(255.0) / np.float32(208.0)
+    lo = np.nextafter(hi, np.float32(0))
+    plo, phi = lo * np.float32(208.0), hi * np.float32(208.0)
+    assert plo < 255.0 <= phi
+    img = oracle.render_rgba(np.array([lo, hi, 0, 0], np.float32), np.full(4, FLUID, np.uint8), 2)
+    assert tuple(img[0]) == (int(plo), 88, 1, 1) and int(plo) == 254
+    assert tuple(img[1]) == (255, 88, 1, 1)
+
+
+def test_render_pixel_order_and_custom_colours():
+    """Pixel (x, y) is byte 4*(x + y*size) (image::ImageBuffer's row-major
+    order over idx(x, y)); the colours come from the arguments."""
+    size = 5
+    d = np.zeros(size * size, np.float32)
+    m = np.full(size * size, FLUID, np.uint8)
+    d[oracle.idx(3, 1, size)] = 2.0
+    m[oracle.idx(0, 4, size)] = WALL
+    img = oracle.render_rgba(d, m, size, fluid=(10, 20, 30, 40), world=(1, 2, 3, 4), obstacle=(9, 8, 7, 6))
+    assert tuple(img[3 + 1 * size]) == (20, 20, 2, 1)
+    assert tuple(img[0 + 4 * size]) == (9, 8, 7, 6)
+    rest = np.ones(size * size, bool)
+    rest[[3 + size, 4 * size]] = False
+    assert (img[rest] == np.array([1, 2, 3, 4], np.uint8)).all()
+    # fluid red 0: inf * 0 = NaN -> 0
+    img = oracle.render_rgba(np.full(4, np.inf, np.float32), np.full(4, FLUID, np.uint8), 2, fluid=(0, 5, 5, 5))
+    assert (img == np.array([0, 5, 255, 1], np.uint8)).all()
