@@ -24,6 +24,8 @@
 
 #include "eqf_internal.h"
 
+
+
 namespace eqf {
 namespace rb {
 
@@ -80,8 +82,10 @@ enum BandSel : uint32_t {
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
 __global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict__ dp, const float* __restrict__ src,
-                                                     float* __restrict__ dst, const float* __restrict__ x0g, float a,
-                                                     float cr, int nsweeps) {
+                                                     float* __restrict__ dst, const float* __restrict__ x0g,
+                                                     const uint8_t* __restrict__ code,
+                                                     const uint8_t* __restrict__ rowf, float a, float cr,
+                                                     int nsweeps) {
   constexpr int H = 3 * T, W = 128, RH = NW * R;
   constexpr int TX = W - 2 * H, TY = RH - 2 * H;
   static_assert(TX > 0 && TY > 0 && TX % 2 == 0 && TY % 2 == 0 && R % 2 == 0, "band geometry");
@@ -90,15 +94,39 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict
   const int S = d.S, N = d.N;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int gx0 = blockIdx.x * TX - H;  // region origin; gx0 + gy0 is even (TX, TY, R even)
-  const int gyb = blockIdx.y * TY - H;
+  // Tile order: workgroups are dealt to the 8 XCDs round-robin by linear id;
+  // XCD k takes a contiguous run of row-major tiles instead, so tiles that
+  // share halo columns run together on one L2 (halves the HBM fetch).
+  int bx, by;
+  {
+    const int ntx = gridDim.x, nt = gridDim.x * gridDim.y;
+    const int L = blockIdx.x + blockIdx.y * ntx;
+    const int per = nt >> 3, rem = nt & 7, xcd = L & 7;
+    const int t = xcd * per + min(xcd, rem) + (L >> 3);
+    bx = t % ntx;
+    by = t / ntx;
+  }
+  const int gx0 = bx * TX - H;  // region origin; gx0 + gy0 is even (TX, TY, R even)
+  const int gyb = by * TY - H;
   const int gy0 = gyb + w * R;          // this wave's band
   const int gxa = gx0 + 2 * lane;
-  const uint8_t* __restrict__ code = d.code;
-  const uint8_t* __restrict__ rowf = d.rowf;
-
   float x[R][2], x0[R][2];
   uint32_t cd[R];
+  // Loads first, all of them, from clamped in-grid addresses (cells outside
+  // the grid are replaced by 0 afterwards), so that one wait covers every
+  // load of the region.
+  uint32_t craw[R][2];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int64_t row = (int64_t)min(max(gy0 + i, 0), S - 1) * S;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int64_t g = row + min(max(gxa + b, 0), S - 1);
+      x[i][b] = src[g];
+      x0[i][b] = x0g[g];
+      if constexpr (MODE != M_PASSIVE) craw[i][b] = code[g];
+    }
+  }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int gy = gy0 + i;
@@ -107,23 +135,25 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict
     for (int b = 0; b < 2; ++b) {
       const int gx = gxa + b;
       const bool in = gx >= 0 && gx < S && gy >= 0 && gy < S;
-      const int64_t g = (int64_t)gy * S + gx;
-      x[i][b] = in ? src[g] : 0.0f;
-      x0[i][b] = in ? x0g[g] : 0.0f;
+      if (!in) {
+        x[i][b] = 0.0f;
+        x0[i][b] = 0.0f;
+      }
       uint32_t sel = 0;
-      if (in) {
-        if (gx >= 1 && gx <= N && gy >= 1 && gy <= N) {
-          const uint32_t c = code[g];
-          sel = BS_UPD;
+      if (gx >= 1 && gx <= N && gy >= 1 && gy <= N) {
+        sel = BS_UPD;
+        if constexpr (MODE != M_PASSIVE) {
+          const uint32_t c = craw[i][b];
           if (MODE == M_ROW && (c & C_FLUID)) sel |= (c & C_RWALL) ? BS_NEG_RIGHT : (c & C_LWALL) ? BS_NEG_LEFT : 0;
           if (MODE == M_COL && (c & C_FLUID)) sel |= (c & C_UWALL) ? BS_NEG_UP : (c & C_DWALL) ? BS_NEG_DOWN : 0;
-        } else if (MODE == M_PASSIVE) {
-          // a ring cell of column i / row j is rewritten iff that column / row has a NoWall cell
-          if (gy == 0 && gx >= 1 && gx <= N) sel = (code[S + gx] & C_COLF) ? BS_DOWN : 0;
-          else if (gy == S - 1 && gx >= 1 && gx <= N) sel = (code[(int64_t)(S - 2) * S + gx] & C_COLF) ? BS_UP : 0;
-          else if (gx == 0 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_RIGHT : 0;
-          else if (gx == S - 1 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_LEFT : 0;
         }
+      } else if (MODE == M_PASSIVE && in) {
+        // a ring cell of column i / row j is rewritten iff that column / row
+        // has a NoWall cell (ring cells only: a rare branch)
+        if (gy == 0 && gx >= 1 && gx <= N) sel = (code[S + gx] & C_COLF) ? BS_DOWN : 0;
+        else if (gy == S - 1 && gx >= 1 && gx <= N) sel = (code[(int64_t)(S - 2) * S + gx] & C_COLF) ? BS_UP : 0;
+        else if (gx == 0 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_RIGHT : 0;
+        else if (gx == S - 1 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_LEFT : 0;
       }
       c2 |= sel << (8 * b);
     }
@@ -390,28 +420,31 @@ struct BandCfg {
 static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}};
 
 template <int MODE, int T, int NW, int R>
-static void launch_band(hipStream_t st, const SceneDesc* dd, const float* src, float* dst, const float* x0, float a,
-                        float cr, int nsw, int S) {
+static void launch_band(hipStream_t st, const SceneDesc* dd, const float* src, float* dst, const float* x0,
+                        const uint8_t* code, const uint8_t* rowf, float a, float cr, int nsw, int S) {
   constexpr int TX = 128 - 6 * T, TY = NW * R - 6 * T;
   dim3 grid((S + TX - 1) / TX, (S + TY - 1) / TY);
-  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), grid, dim3(64 * NW), 0, st, dd, src, dst, x0, a, cr, nsw);
+  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), grid, dim3(64 * NW), 0, st, dd, src, dst, x0, code, rowf, a, cr,
+                     nsw);
 }
 
 template <int MODE, int NW, int R>
 static void launch_band_t(hipStream_t st, int T, const SceneDesc* dd, const float* src, float* dst,
-                          const float* x0, float a, float cr, int nsw, int S) {
-  if (T == 1) launch_band<MODE, 1, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
-  else if (T == 2) launch_band<MODE, 2, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
-  else if (T == 4) launch_band<MODE, 4, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
-  else launch_band<MODE, 8, NW, R>(st, dd, src, dst, x0, a, cr, nsw, S);
+                          const float* x0, const uint8_t* code, const uint8_t* rowf, float a, float cr, int nsw,
+                          int S) {
+  if (T == 1) launch_band<MODE, 1, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else if (T == 2) launch_band<MODE, 2, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else if (T == 4) launch_band<MODE, 4, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else launch_band<MODE, 8, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
 }
 
 template <int MODE>
 static void launch_gs_mode(hipStream_t st, int tile, int T, const SceneDesc* dd, const float* src, float* dst,
-                           const float* x0, float a, float cr, int nsw, int S) {
-  if (tile == 1) launch_band_t<MODE, 8, 32>(st, T, dd, src, dst, x0, a, cr, nsw, S);
-  else if (tile == 2) launch_band_t<MODE, 16, 16>(st, T, dd, src, dst, x0, a, cr, nsw, S);
-  else launch_band_t<MODE, 8, 16>(st, T, dd, src, dst, x0, a, cr, nsw, S);
+                           const float* x0, const uint8_t* code, const uint8_t* rowf, float a, float cr, int nsw,
+                           int S) {
+  if (tile == 1) launch_band_t<MODE, 8, 32>(st, T, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else if (tile == 2) launch_band_t<MODE, 16, 16>(st, T, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else launch_band_t<MODE, 8, 16>(st, T, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
 }
 
 // lin_solve in red-black order: ceil(iters/T) tile passes, ping-ponging
@@ -425,9 +458,10 @@ static void rb_lin_solve(eqf_scene* s, RBWork* w, int mode, float* x, const floa
   float* b_ = w->tmp;
   for (int done = 0; done < iters;) {
     const int nsw = std::min(T, iters - done);
-    if (mode == M_ROW) launch_gs_mode<M_ROW>(s->stream, w->tile, T, dd, a_, b_, x0, a, cr, nsw, S);
-    else if (mode == M_COL) launch_gs_mode<M_COL>(s->stream, w->tile, T, dd, a_, b_, x0, a, cr, nsw, S);
-    else launch_gs_mode<M_PASSIVE>(s->stream, w->tile, T, dd, a_, b_, x0, a, cr, nsw, S);
+    const uint8_t *cd = s->d_code, *rf = s->d_rowf;
+    if (mode == M_ROW) launch_gs_mode<M_ROW>(s->stream, w->tile, T, dd, a_, b_, x0, cd, rf, a, cr, nsw, S);
+    else if (mode == M_COL) launch_gs_mode<M_COL>(s->stream, w->tile, T, dd, a_, b_, x0, cd, rf, a, cr, nsw, S);
+    else launch_gs_mode<M_PASSIVE>(s->stream, w->tile, T, dd, a_, b_, x0, cd, rf, a, cr, nsw, S);
     std::swap(a_, b_);
     done += nsw;
   }

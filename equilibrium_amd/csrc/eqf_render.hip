@@ -26,24 +26,35 @@ __device__ __forceinline__ uint32_t rust_u8(float f) {
   return (uint32_t)f;
 }
 
-// One pixel per thread: rgba[e] for cell e = x + y*S of the row-major
-// density `d` and mask `m` (renderer_helpers.rs:143-165).
+__device__ __forceinline__ uint32_t pixel(float dens, uint32_t cell, uint32_t fluid, uint32_t world,
+                                          uint32_t obstacle) {
+  if (cell == EQF_DEFAULT_WALL) return obstacle;
+  // [(density * fluid.r as f32) as u8, fluid.g, density as u8, 1]
+  if (dens != 0.0f && cell == EQF_NO_WALL)
+    return rust_u8(dens * (float)(fluid & 0xffu)) | (fluid & 0xff00u) | (rust_u8(dens) << 16) | (1u << 24);
+  return world;
+}
+
+// Four pixels per thread (16 B density, 4 B mask in, 16 B RGBA out):
+// rgba[e] for cells e = x + y*S of the row-major density `d` and mask `m`
+// (renderer_helpers.rs:143-165).  `d`, `m` and `rgba` are 16-byte aligned;
+// the last thread takes the n % 4 tail one pixel at a time.
 __global__ void k_render(int64_t n, const float* __restrict__ d, const uint8_t* __restrict__ m, uint32_t fluid,
                          uint32_t world, uint32_t obstacle, uint32_t* __restrict__ rgba) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const uint32_t cell = m[e];
-  const float dens = d[e];
-  uint32_t px;
-  if (cell == EQF_DEFAULT_WALL) {
-    px = obstacle;
-  } else if (dens != 0.0f && cell == EQF_NO_WALL) {
-    // [(density * fluid.r as f32) as u8, fluid.g, density as u8, 1]
-    px = rust_u8(dens * (float)(fluid & 0xffu)) | (fluid & 0xff00u) | (rust_u8(dens) << 16) | (1u << 24);
+  const int64_t e0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+  if (e0 >= n) return;
+  if (e0 + 4 <= n) {
+    const float4 dv = *reinterpret_cast<const float4*>(d + e0);
+    const uint32_t mv = *reinterpret_cast<const uint32_t*>(m + e0);
+    uint4 o;
+    o.x = pixel(dv.x, mv & 0xffu, fluid, world, obstacle);
+    o.y = pixel(dv.y, (mv >> 8) & 0xffu, fluid, world, obstacle);
+    o.z = pixel(dv.z, (mv >> 16) & 0xffu, fluid, world, obstacle);
+    o.w = pixel(dv.w, mv >> 24, fluid, world, obstacle);
+    *reinterpret_cast<uint4*>(rgba + e0) = o;
   } else {
-    px = world;
+    for (int64_t e = e0; e < n; ++e) rgba[e] = pixel(d[e], m[e], fluid, world, obstacle);
   }
-  rgba[e] = px;
 }
 
 uint32_t pack(const uint8_t c[4]) {
@@ -51,6 +62,9 @@ uint32_t pack(const uint8_t c[4]) {
 }
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// RGBA follows the density in one staging buffer, at a 16-byte boundary.
+size_t rgba_offset_floats(size_t n) { return (n + 3) & ~(size_t)3; }
 
 }  // namespace
 
@@ -68,7 +82,7 @@ static int render_inputs(eqf_scene* s, float* d_dens_rm) {
 
 static int launch_render(eqf_scene* s, const float* d_dens_rm, const eqf_colors* c, uint8_t* d_rgba) {
   const int64_t n = (int64_t)s->S * s->S;
-  hipLaunchKernelGGL(k_render, dim3(blocks_for(n)), dim3(256), 0, s->stream, n, d_dens_rm, s->d_mask,
+  hipLaunchKernelGGL(k_render, dim3(blocks_for((n + 3) / 4)), dim3(256), 0, s->stream, n, d_dens_rm, s->d_mask,
                      pack(c->fluid), pack(c->world), pack(c->obstacle), (uint32_t*)d_rgba);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? EQF_OK : eqf_internal_error(EQF_EDEVICE, std::string("render: ") + hipGetErrorString(e));
@@ -96,9 +110,9 @@ extern "C" int eqf_render_rgba(eqf_scene* s, const eqf_colors* c, uint8_t* rgba_
   const size_t n = (size_t)s->S * s->S;
   float* d_dens = nullptr;
   uint8_t* d_rgba = nullptr;
-  if (e == hipSuccess) e = hipMalloc(&d_dens, sizeof(float) * n + 4 * n);
+  if (e == hipSuccess) e = hipMalloc(&d_dens, sizeof(float) * rgba_offset_floats(n) + 4 * n);
   if (e != hipSuccess) return eqf_internal_error(EQF_ENOMEM, std::string("render buffer: ") + hipGetErrorString(e));
-  d_rgba = (uint8_t*)(d_dens + n);
+  d_rgba = (uint8_t*)(d_dens + rgba_offset_floats(n));
   int rc = render_inputs(s, d_dens);
   if (!rc) rc = launch_render(s, d_dens, c, d_rgba);
   if (!rc) {
@@ -152,7 +166,7 @@ extern "C" int eqf_snapshot_create(eqf_scene* s, int depth, int with_rgba, const
   if (c) sn->colors = *c;
   else eqf_colors_default(&sn->colors);
   const size_t n = (size_t)s->S * s->S;
-  const size_t bytes = sizeof(float) * n + (sn->with_rgba ? 4 * n : 0);
+  const size_t bytes = sn->with_rgba ? sizeof(float) * rgba_offset_floats(n) + 4 * n : sizeof(float) * n;
   if (e == hipSuccess) e = hipStreamCreate(&sn->copy);
   sn->slots.resize(depth);
   for (auto& sl : sn->slots) {
@@ -161,8 +175,8 @@ extern "C" int eqf_snapshot_create(eqf_scene* s, int depth, int with_rgba, const
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess && sn->with_rgba) {
-      sl.d_rgba = (uint8_t*)(sl.d_dens + n);
-      sl.h_rgba = (uint8_t*)(sl.h_dens + n);
+      sl.d_rgba = (uint8_t*)(sl.d_dens + rgba_offset_floats(n));
+      sl.h_rgba = (uint8_t*)(sl.h_dens + rgba_offset_floats(n));
     }
   }
   if (e != hipSuccess) {
@@ -185,7 +199,7 @@ extern "C" int eqf_snapshot_capture(eqf_snapshot* sn, int64_t frame) {
   int rc = render_inputs(s, sl.d_dens);
   if (!rc && sn->with_rgba) rc = launch_render(s, sl.d_dens, &sn->colors, sl.d_rgba);
   if (rc) return rc;
-  const size_t bytes = sizeof(float) * n + (sn->with_rgba ? 4 * n : 0);
+  const size_t bytes = sn->with_rgba ? sizeof(float) * rgba_offset_floats(n) + 4 * n : sizeof(float) * n;
   e = hipEventRecord(sl.ready, s->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(sn->copy, sl.ready, 0);
   if (e == hipSuccess) e = hipMemcpyAsync(sl.h_dens, sl.d_dens, bytes, hipMemcpyDeviceToHost, sn->copy);
