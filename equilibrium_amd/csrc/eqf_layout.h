@@ -50,8 +50,8 @@ enum CodeBits : uint32_t {
   C_COLF = 64u,     // column c holds at least one NoWall cell (Passive ring copy)
 };
 
-// Boundary selectors of a CYCLIC scene: one byte per interior cell and mode
-// (array of mode m at byte m * ns * P * 64, cells at cyc_code_off), telling
+// Boundary selectors of a CYCLIC scene: two bits per interior cell and mode
+// (array of mode m at byte m * cyc_code_bytes, cells at cyc_code_word), telling
 // the wavefront what set_boundaries of that mode makes of the cell:
 //   M_ROW:     0 keep, 1 -(left neighbour), 2 -(right neighbour)  (right wins, fluid.rs:151-164)
 //   M_COL:     0 keep, 1 -(upper neighbour), 2 -(lower neighbour) (up wins, fluid.rs:165-177);
@@ -115,15 +115,23 @@ EQF_HD inline int64_t cyc_off(int x, int y, int S, int P, int ns) {
   return ((int64_t)s * P + t) * 64 + r;
 }
 
-// Boundary selectors of a cyclic scene: one byte per interior cell and mode,
-// with the four slots 4k..4k+3 of a lane packed in one 32-bit word, so a wave
-// reads the selectors of four wavefront steps with one load: slot t of lane r
-// in strip s is byte (s*P + (t & ~3))*64 + 4*r + (t & 3) of the mode's array.
-EQF_HD inline int64_t cyc_code_off(int x, int y, int P) {
+// Boundary selectors of a cyclic scene: two bits per interior cell and mode,
+// the sixteen slots 16k..16k+15 of a lane packed in one 32-bit word, so a
+// wave reads the selectors of a whole 16-step wavefront block with one load:
+// slot t of lane r in strip s is bits 2*(t & 15) of word (s*P/16 + t/16)*64 + r
+// of the mode's array (cyc_code_bytes bytes per mode).
+EQF_HD inline int64_t cyc_code_bytes(int P, int ns) { return (int64_t)ns * P * 16; }
+EQF_HD inline int64_t cyc_code_word(int x, int y, int P) {
   const int j = y - 1, s = j >> 6, r = j & 63;
   int t = x - 1 + r;
   if (t >= P) t -= P;
-  return ((int64_t)s * P + (t & ~3)) * 64 + 4 * r + (t & 3);
+  return ((int64_t)s * (P >> 4) + (t >> 4)) * 64 + r;
+}
+EQF_HD inline int cyc_code_shift(int x, int y, int P) {
+  const int r = (y - 1) & 63;
+  int t = x - 1 + r;
+  if (t >= P) t -= P;
+  return 2 * (t & 15);
 }
 
 EQF_HD inline int64_t cyc_field_floats(int S, int P, int ns) {

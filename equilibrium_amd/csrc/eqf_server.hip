@@ -84,7 +84,7 @@ struct eqf_server {
 static uint64_t estimate_bytes(const eqf_params& p) {
   const int S = (int)p.size, N = S - 2, ns = (N + 63) / 64, P = cyc_period(N);
   const uint64_t fs = (uint64_t)cyc_field_floats(S, P, ns);
-  return (NFIELDS + 3) * fs * 4 + 3ull * ns * P * 64 + (uint64_t)S + (uint64_t)N * 96 + sizeof(SceneDesc) + 4096;
+  return (NFIELDS + 3) * fs * 4 + 3ull * (uint64_t)cyc_code_bytes(P, ns) + (uint64_t)S + (uint64_t)N * 96 + sizeof(SceneDesc) + 4096;
 }
 
 static int admit(eqf_server* sv, Req* r, std::string& err) {

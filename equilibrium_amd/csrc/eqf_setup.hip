@@ -106,7 +106,8 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (colf[x]) c |= C_COLF;
     if (fill && g.layout == L_ROWMAJOR) code[off_of(g, x, y)] = (uint8_t)c;
     if (fill && g.layout == L_CYCLIC) {
-      const int64_t o = cyc_code_off(x, y, g.P), mstride = (int64_t)g.ns * g.P * 64;
+      const int64_t o = cyc_code_word(x, y, g.P), mstride = cyc_code_bytes(g.P, g.ns) / 4;
+      const int sh = cyc_code_shift(x, y, g.P);
       const bool strip_bottom = (y & 63) == 0 && y < N;  // the row below is in the next strip
       uint8_t srow = SEL_KEEP, scol = SEL_KEEP;
       if (fl) {
@@ -115,9 +116,11 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
       } else if (((y - 1) & 63) == 0 && y > 1 && !wall(x, y - 1) && !wall(x, y - 2)) {
         scol = SEL_FIXUP;
       }
-      code[o] = srow;
-      code[mstride + o] = scol;
-      code[2 * mstride + o] = colf[x] ? 1 : 0;
+      // the words start at zero; OR in the non-zero selectors
+      uint32_t* cw = reinterpret_cast<uint32_t*>(code);
+      if (srow) atomicOr(&cw[o], (uint32_t)srow << sh);
+      if (scol) atomicOr(&cw[mstride + o], (uint32_t)scol << sh);
+      if (colf[x]) atomicOr(&cw[2 * mstride + o], 1u << sh);
     }
     if (fl) {
       int sx = -1;
@@ -254,7 +257,7 @@ int eqf_dev_rebuild_bnd(eqf_scene* s) {
   }
   e = hipMemsetAsync(d_cnt, 0, 3 * sizeof(int), s->stream);
   if (e == hipSuccess) {
-    const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)s->ns * s->P * 64 : (size_t)n;
+    const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)cyc_code_bytes(s->P, s->ns) : (size_t)n;
     e = hipMemsetAsync(s->d_code, 0, ncode, s->stream);
   }
   if (e == hipSuccess) {

@@ -172,7 +172,7 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 template <int MODE, int D, int B>
 __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict__ H, gf* __restrict__ X,
                                              gcf* __restrict__ X0, float a, float cr, int iters) {
-  static_assert(16 % B == 0 && D < B, "B must divide 16 and exceed the prefetch depth");
+  static_assert(B == 16 && D < B, "blocks of 16 steps (one selector word), deeper than the prefetch");
   constexpr int L = (64 + B + D + B - 1) / B * B;  // strip lag, a multiple of B
   constexpr int LAG = 64;                          // sweep lag between streams of a strip
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
@@ -241,7 +241,8 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const uint32_t fbytes = (uint32_t)sc.fstride * 4u;
   const rsrc_t rX = make_rsrc(X, fbytes);
   const rsrc_t rX0 = make_rsrc(X0, fbytes);
-  const rsrc_t rC = make_rsrc((gcu8*)sc.code + MODE * RO, (uint32_t)RO);  // this mode's selectors
+  const int CB = (int)cyc_code_bytes(P, ns);
+  const rsrc_t rC = make_rsrc((gcu8*)sc.code + MODE * CB, (uint32_t)CB);  // this mode's selectors
   const int OOB = (int)0x80000000;  // voffset that drops a store / reads 0
   const int vl = lane * 4;
   const int vls = valid ? vl : OOB;                   // stores: rows past N stay untouched
@@ -260,13 +261,13 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const bool raw_ok = last && valid && has_next;      // lane 63 hands its raw values down
   const bool post_ok = first && has_prev;             // lane 0 hands its final values up
   const int sb = uni(s * P * 256);                        // strip base in X (bytes)
-  const int cb = uni(s * P * 64);                        // strip base in the code array (bytes)
+  const int cb = uni(s * P * 16);                        // strip base in the code array (bytes)
   const int fxb = uni(has_prev ? sb - P * 256 + 252 : 0);  // lane 63 of the strip above (bytes)
   const int itP = uni(nq * P);  // positions of this stream
   const int o64 = uni(64 % P);
 
   float fxs[D], fx0[D], fe[D], fe2[D];  // fe / fe2: the edge value, as the up / down shift's edge operand
-  uint32_t fcw[4];  // packed codes of steps 4k..4k+3 in fcw[k & 3], loaded 13-16 steps ahead
+  uint32_t fcw, fcw_n;  // selector words of this block and of the next one
 
   // Stream j of strip s idles for its first (s*L + j*LAG)/B blocks (every
   // lane before its first sweep): those blocks are barriers only.  Its first
@@ -297,8 +298,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
       fe[i] = H[e_row + i];
       fe2[i] = H[e_row + i];
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) fcw[k] = ldu32(rC, vl + k * 256, cb);  // slots 0..15
+    fcw = ldu32(rC, vl, cb);  // slots 0..15
     // xself of the first step: what the step before would have loaded (slot
     // 0, lane 0's own column-1 cell, from the previous sweep when jst > 0)
     xself = ldf(rX, vl, sb);
