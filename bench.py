@@ -266,14 +266,20 @@ def bench_snapshots(frames, depth=2, N=4096, tiling=None):
             "sync_overhead_frac": ms["sync_download_density"] / ms["no_readback"] - 1.0}
 
 
-def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
-    """Poisson arrivals of scene requests (N log-uniform in
-    [64, n_max], frames U(50,500), sources and obstacles as in small_scene),
-    admitted against an HBM budget; each rate runs `seconds` of arrivals,
-    then the remaining requests are cancelled.  p50/p99 frame latency =
-    tick durations weighted by resident scenes.  To keep the client on the
-    Poisson schedule, N is rounded to one of `buckets` log-spaced sizes whose
-    scenes (sources, obstacles) are generated before the clock starts."""
+WAVEFRONT_MAX_N = 1024  # exact mode holds up to 16 strips of 64 rows per workgroup
+
+
+def bench_serving(rates, seconds, budget_gb=200.0, n_max=2048, buckets=16):
+    """Poisson arrivals of scene requests (N log-uniform in [64, n_max],
+    frames U(50,500), sources and obstacles as in small_scene), admitted
+    against an HBM budget; each rate runs `seconds` of arrivals, then the
+    remaining requests are cancelled.  Scenes up to N=1024 run in exact
+    wavefront mode (one fused launch per tick), larger ones in red-black mode
+    (their frame graphs beside it).  p50/p99 frame latency = tick durations
+    weighted by resident scenes.  To keep the client on the Poisson schedule,
+    N is rounded to one of `buckets` log-spaced sizes whose scenes (sources,
+    obstacles) are generated before the clock starts and lent to the server
+    (read in place at admission)."""
     import math
     sizes = sorted({int(round(64 * (n_max / 64) ** (k / (buckets - 1)))) for k in range(buckets)})
     pool = {N: scenes.stam_scene(N, scenes.derive_seed(40, N), n_rects=1) for N in sizes}
@@ -290,7 +296,8 @@ def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
                 time.sleep(0.0002)
             lag = max(lag, time.perf_counter() - t0 - ta)
             nb = sizes[min(range(len(sizes)), key=lambda k: abs(logs[k] - math.log(N)))]
-            tickets.append(srv.submit(pool[nb], nframes=frames))
+            mode = _abi.GS_WAVEFRONT if nb <= WAVEFRONT_MAX_N else _abi.GS_REDBLACK
+            tickets.append(srv.submit(pool[nb], nframes=frames, gs_mode=mode, borrow=True))
         t_end = time.perf_counter() - t0
         time.sleep(0.5)
         st = srv.stats()
@@ -303,7 +310,8 @@ def bench_serving(rates, seconds, budget_gb=200.0, n_max=1024, buckets=16):
              "p50_frame_ms": st["p50_frame_ms"], "p99_frame_ms": st["p99_frame_ms"],
              "p50_queue_ms": st["p50_queue_ms"], "p99_queue_ms": st["p99_queue_ms"],
              "scene_frames": st["scene_frames"], "ticks": st["ticks"],
-             "max_resident": st["max_resident"], "peak_gb": st["peak_bytes"] / 2 ** 30}
+             "max_resident": st["max_resident"], "peak_gb": st["peak_bytes"] / 2 ** 30,
+             "red_black_share": sum(1 for r in reqs if r[1] > WAVEFRONT_MAX_N) / max(1, len(reqs))}
         log(f"serve rate {rate}/s: {r}")
         srv.close()
         out.append(r)

@@ -47,7 +47,8 @@ class ServerConfig(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("params", Params), ("nframes", ctypes.c_int64), ("n_rects", ctypes.c_int32),
-                ("keep_velocity", ctypes.c_int32), ("rects", ctypes.c_void_p),
+                ("keep_velocity", ctypes.c_int32), ("borrow_inputs", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("rects", ctypes.c_void_p),
                 ("src_density", ctypes.c_void_p), ("src_vx", ctypes.c_void_p),
                 ("src_vy", ctypes.c_void_p), ("init_vx", ctypes.c_void_p), ("init_vy", ctypes.c_void_p)]
 
@@ -379,11 +380,14 @@ class Server:
         check(lib().eqf_server_create(self.ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
         self._keep = {}
+        self._borrowed = {}
 
-    def submit(self, sc, nframes=None, keep_velocity=False):
-        """Submits a scene dict (equilibrium_amd.scenes format)."""
+    def submit(self, sc, nframes=None, keep_velocity=False, gs_mode=GS_WAVEFRONT, borrow=False):
+        """Submits a scene dict (equilibrium_amd.scenes format).  borrow=True
+        lets the server read the scene's arrays in place at admission (this
+        object keeps them referenced until close())."""
         from . import scenes as _sc
-        p = _sc.scene_params(sc)
+        p = _sc.scene_params(sc, gs_mode)
         S = sc["size"]
         arrs = {}
         for k in ("src_d", "src_u", "src_v", "init_u", "init_v"):
@@ -391,12 +395,15 @@ class Server:
             arrs[k] = None if a is None else np.ascontiguousarray(a, np.float32).ravel()
         rects = np.ascontiguousarray(np.array(sc["rects"], np.int64).reshape(-1, 4))
         req = Request(p, sc["frames"] if nframes is None else nframes, len(sc["rects"]),
-                      1 if keep_velocity else 0, rects.ctypes.data if len(sc["rects"]) else None,
+                      1 if keep_velocity else 0, 1 if borrow else 0, 0,
+                      rects.ctypes.data if len(sc["rects"]) else None,
                       *(None if arrs[k] is None else arrs[k].ctypes.data
                         for k in ("src_d", "src_u", "src_v", "init_u", "init_v")))
         t = ctypes.c_uint64()
         check(lib().eqf_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
         self._keep[t.value] = S
+        if borrow:
+            self._borrowed[t.value] = (rects, arrs)
         return t.value
 
     def poll(self, ticket):
@@ -425,6 +432,7 @@ class Server:
         if getattr(self, "h", None):
             check(lib().eqf_server_destroy(self.h))
             self.h = None
+        self._borrowed = {}
 
     def __del__(self):
         try:

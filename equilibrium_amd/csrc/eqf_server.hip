@@ -1,8 +1,11 @@
 // eqf_server.hip -- continuous batching of scene requests on one GPU.
 //
 // A server thread owns every admitted scene.  Each tick it launches ONE fused
-// frame kernel (eqf_frames_cyclic, grid = resident scenes) that advances every
-// resident scene by one frame (fluid.rs:437-524).  While that launch runs,
+// frame kernel (eqf_frames_cyclic, grid = resident wavefront-mode scenes) that
+// advances every such scene by one frame (fluid.rs:437-524); red-black scenes
+// (any size, for grids past the wavefront's 1026 limit) advance by one
+// replay of their frame graph each, on a second stream beside it.  While the
+// tick runs,
 // the thread admits queued requests whose device memory fits the budget:
 // it allocates them, writes Fluid::new's state, marks obstacles and uploads
 // sources on each scene's own stream, so admission overlaps the running tick.
@@ -35,13 +38,16 @@ struct Req {
   eqf_params p{};
   int64_t nframes = 0;
   std::vector<int64_t> rects;
-  std::vector<float> sd, su, sv, iu, iv;
+  std::vector<float> sd, su, sv, iu, iv;  // copied inputs
+  // inputs read in place (borrow_inputs): the caller keeps them valid until admission
+  const float *bsd = nullptr, *bsu = nullptr, *bsv = nullptr, *biu = nullptr, *biv = nullptr;
   bool keep_vel = false;
   int state = EQF_REQ_QUEUED;
   bool cancel = false;
   int64_t frames_done = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
   eqf_scene* scene = nullptr;
+  bool rb = false;  // red-black mode: own frame graph on the red-black stream
   uint64_t bytes = 0;
   int64_t cells = 0;
   std::vector<float> out_d, out_u, out_v;
@@ -65,6 +71,11 @@ struct eqf_server {
   std::thread th;
   hipStream_t stream = nullptr;   // ticks
   hipStream_t astream = nullptr;  // admissions (overlap the running tick)
+  // red-black scenes' frames, beside the fused launch; two streams so that
+  // the frames of two scenes overlap (small kernels, partly filled grids)
+  static constexpr int kRbStreams = 2;
+  hipStream_t rstream[kRbStreams] = {nullptr, nullptr};
+  int rb_next = 0;
   SceneDesc* d_descs = nullptr;
   int desc_cap = 0;
   Clock::time_point t0;
@@ -80,11 +91,18 @@ struct eqf_server {
   }
 };
 
-// Device bytes of a wavefront scene of this size with frame sources.
+// Device bytes of a scene of this size and mode with frame sources.
 static uint64_t estimate_bytes(const eqf_params& p) {
   const int S = (int)p.size, N = S - 2, ns = (N + 63) / 64, P = cyc_period(N);
+  if (p.gs_mode == EQF_GS_REDBLACK) {
+    // fields + sources, rule bytes, ping-pong field, row flags, break columns, lists
+    const uint64_t fs = (uint64_t)rm_field_floats(S), cells = (uint64_t)S * S;
+    return (NFIELDS + 3) * fs * 4 + cells + cells * 4 + (uint64_t)S * 5 + (uint64_t)N * 96 + sizeof(SceneDesc) +
+           8192;
+  }
   const uint64_t fs = (uint64_t)cyc_field_floats(S, P, ns);
-  return (NFIELDS + 3) * fs * 4 + 3ull * (uint64_t)cyc_code_bytes(P, ns) + (uint64_t)S + (uint64_t)N * 96 + sizeof(SceneDesc) + 4096;
+  return (NFIELDS + 3) * fs * 4 + 3ull * (uint64_t)cyc_code_bytes(P, ns) + (uint64_t)S + (uint64_t)N * 96 +
+         sizeof(SceneDesc) + 4096;
 }
 
 static int admit(eqf_server* sv, Req* r, std::string& err) {
@@ -92,11 +110,14 @@ static int admit(eqf_server* sv, Req* r, std::string& err) {
   int rc = eqf_internal_scene_create(sv->ctx, &r->p, sv->astream, &s);
   for (size_t k = 0; !rc && k + 4 <= r->rects.size(); k += 4)
     rc = eqf_fill_rect(s, r->rects[k], r->rects[k + 1], r->rects[k + 2], r->rects[k + 3]);
-  if (!rc && !r->iu.empty()) rc = eqf_upload(s, EQF_VX, r->iu.data());
-  if (!rc && !r->iv.empty()) rc = eqf_upload(s, EQF_VY, r->iv.data());
-  if (!rc && !r->sd.empty()) rc = eqf_set_frame_source(s, EQF_DENSITY, r->sd.data());
-  if (!rc && !r->su.empty()) rc = eqf_set_frame_source(s, EQF_VX, r->su.data());
-  if (!rc && !r->sv.empty()) rc = eqf_set_frame_source(s, EQF_VY, r->sv.data());
+  auto in = [](const std::vector<float>& v, const float* b) { return v.empty() ? b : v.data(); };
+  const float *iu = in(r->iu, r->biu), *iv = in(r->iv, r->biv);
+  const float *sd = in(r->sd, r->bsd), *su = in(r->su, r->bsu), *svv = in(r->sv, r->bsv);
+  if (!rc && iu) rc = eqf_upload(s, EQF_VX, iu);
+  if (!rc && iv) rc = eqf_upload(s, EQF_VY, iv);
+  if (!rc && sd) rc = eqf_set_frame_source(s, EQF_DENSITY, sd);
+  if (!rc && su) rc = eqf_set_frame_source(s, EQF_VX, su);
+  if (!rc && svv) rc = eqf_set_frame_source(s, EQF_VY, svv);
   if (!rc) rc = eqf_internal_prepare(s);
   if (!rc) rc = eqf_sync(s);
   if (rc) {
@@ -104,9 +125,15 @@ static int admit(eqf_server* sv, Req* r, std::string& err) {
     if (s) eqf_scene_destroy(s);
     return rc;
   }
+  r->rb = s->layout == L_ROWMAJOR;
+  if (r->rb) {  // its frames (and graph) live on a red-black stream
+    s->stream = sv->rstream[sv->rb_next];
+    sv->rb_next = (sv->rb_next + 1) % eqf_server::kRbStreams;
+  }
   r->scene = s;
   r->cells = (int64_t)(r->p.size - 2) * (r->p.size - 2);
-  // release the host copies of the inputs
+  // release the host copies of the inputs (borrowed ones are not read again)
+  r->bsd = r->bsu = r->bsv = r->biu = r->biv = nullptr;
   std::vector<float>().swap(r->sd);
   std::vector<float>().swap(r->su);
   std::vector<float>().swap(r->sv);
@@ -149,7 +176,7 @@ static void server_loop(eqf_server* sv) {
           finish(sv, r);
           lk.lock();
           sv->used_bytes -= r->bytes;
-          r->state = EQF_REQ_CANCELLED;
+          if (r->state != EQF_REQ_FAILED) r->state = EQF_REQ_CANCELLED;
           it = sv->active.erase(it);
         } else {
           ++it;
@@ -157,34 +184,37 @@ static void server_loop(eqf_server* sv) {
       }
       act = sv->active;
     }
-    // Launch one frame of every resident scene (largest first).
-    bool launched = false;
+    // Launch one frame of every resident scene: the wavefront-mode scenes in
+    // one fused launch (largest first), the red-black ones by their graphs.
+    bool launched = false, launched_rb = false;
     double t_launch = sv->now_ms();
-    if (!act.empty()) {
-      std::stable_sort(act.begin(), act.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
+    std::vector<Req*> wf, rbs;
+    for (Req* r : act) (r->rb ? rbs : wf).push_back(r);
+    if (!wf.empty()) {
+      std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
       int maxns = 1;
-      for (Req* r : act) maxns = std::max(maxns, r->scene->ns);
-      if (act != last_list) {
-        descs.resize(act.size());
-        for (size_t i = 0; i < act.size(); ++i) {
-          descs[i] = act[i]->scene->desc;
-          descs[i].prio = eqf_batch_prio(act[i]->scene->ns, maxns);
+      for (Req* r : wf) maxns = std::max(maxns, r->scene->ns);
+      if (wf != last_list) {
+        descs.resize(wf.size());
+        for (size_t i = 0; i < wf.size(); ++i) {
+          descs[i] = wf[i]->scene->desc;
+          descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
         }
-        if ((int)act.size() > sv->desc_cap) {
+        if ((int)wf.size() > sv->desc_cap) {
           (void)hipFree(sv->d_descs);
-          sv->desc_cap = std::max<int>((int)act.size(), sv->desc_cap * 2);
+          sv->desc_cap = std::max<int>((int)wf.size(), sv->desc_cap * 2);
           (void)hipMalloc(&sv->d_descs, sizeof(SceneDesc) * sv->desc_cap);
         }
-        (void)hipMemcpyAsync(sv->d_descs, descs.data(), sizeof(SceneDesc) * act.size(), hipMemcpyHostToDevice,
+        (void)hipMemcpyAsync(sv->d_descs, descs.data(), sizeof(SceneDesc) * wf.size(), hipMemcpyHostToDevice,
                              sv->stream);
-        last_list = act;
+        last_list = wf;
       }
       int lds = 0, waves = 1;
-      for (Req* r : act) {
+      for (Req* r : wf) {
         lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
         waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
       }
-      hipError_t e = eqf_internal_launch(sv->d_descs, (int)act.size(), waves, lds, 1, sv->stream);
+      hipError_t e = eqf_internal_launch(sv->d_descs, (int)wf.size(), waves, lds, 1, sv->stream);
       if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->thread_error = hipGetErrorString(e);
@@ -192,6 +222,23 @@ static void server_loop(eqf_server* sv) {
         launched = true;
       }
     }
+    for (Req* r : rbs) {
+      std::string err;
+      if (rb_step(r->scene, 1, err) != EQF_OK) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->thread_error = err;
+        r->state = EQF_REQ_FAILED;
+        r->cancel = true;  // dropped at the next tick
+      } else {
+        launched_rb = true;
+      }
+    }
+    auto tick_running = [&] {
+      if (launched && hipStreamQuery(sv->stream) == hipErrorNotReady) return true;
+      for (hipStream_t q : sv->rstream)
+        if (launched_rb && hipStreamQuery(q) == hipErrorNotReady) return true;
+      return false;
+    };
     // Admit queued requests while the tick runs: at least one per tick, then
     // more only while the launch is still in flight (with nothing resident,
     // for up to 50 ms), so admission never delays the resident scenes' ticks.
@@ -199,8 +246,8 @@ static void server_loop(eqf_server* sv) {
     const double t_adm0 = sv->now_ms();
     for (;;) {
       if (!admitted.empty()) {
-        if (launched && hipStreamQuery(sv->stream) != hipErrorNotReady) break;
-        if (!launched && sv->now_ms() - t_adm0 > 50.0) break;
+        if ((launched || launched_rb) && !tick_running()) break;
+        if (!launched && !launched_rb && sv->now_ms() - t_adm0 > 50.0) break;
       }
       Req* r = nullptr;
       {
@@ -232,20 +279,23 @@ static void server_loop(eqf_server* sv) {
       admitted.push_back(r);
     }
     double tick_ms = 0;
-    if (launched) {
-      (void)hipStreamSynchronize(sv->stream);
-      tick_ms = sv->now_ms() - t_launch;
-    }
+    if (launched) (void)hipStreamSynchronize(sv->stream);
+    if (launched_rb)
+      for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
+    if (launched || launched_rb) tick_ms = sv->now_ms() - t_launch;
     // Retire finished scenes, then add the admitted ones.
     std::vector<Req*> done;
     {
       std::lock_guard<std::mutex> lk(sv->mu);
-      if (launched) {
+      if (launched || launched_rb) {
+        std::vector<Req*> ran = launched ? wf : std::vector<Req*>();
+        for (Req* r : rbs)
+          if (!r->cancel) ran.push_back(r);
         sv->ticks++;
-        sv->scene_frames += (int64_t)act.size();
+        sv->scene_frames += (int64_t)ran.size();
         sv->busy_ms += tick_ms;
-        sv->tick_lat.emplace_back(tick_ms, (int)act.size());
-        for (Req* r : act) {
+        sv->tick_lat.emplace_back(tick_ms, (int)ran.size());
+        for (Req* r : ran) {
           r->frames_done++;
           sv->cell_updates += (double)r->cells;
         }
@@ -306,8 +356,13 @@ extern "C" int eqf_server_create(eqf_ctx* ctx, const eqf_server_config* cfg, eqf
   sv->ctx = ctx;
   sv->cfg = *cfg;
   sv->t0 = Clock::now();
-  if (hipStreamCreate(&sv->stream) != hipSuccess || hipStreamCreate(&sv->astream) != hipSuccess) {
+  bool ok = hipStreamCreate(&sv->stream) == hipSuccess && hipStreamCreate(&sv->astream) == hipSuccess;
+  for (hipStream_t& q : sv->rstream) ok = ok && hipStreamCreate(&q) == hipSuccess;
+  if (!ok) {
     if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    if (sv->astream) (void)hipStreamDestroy(sv->astream);
+    for (hipStream_t q : sv->rstream)
+      if (q) (void)hipStreamDestroy(q);
     delete sv;
     return eqf_internal_error(EQF_EDEVICE, "hipStreamCreate failed");
   }
@@ -318,8 +373,6 @@ extern "C" int eqf_server_create(eqf_ctx* ctx, const eqf_server_config* cfg, eqf
 
 extern "C" int eqf_server_submit(eqf_server* sv, const eqf_request* req, uint64_t* ticket) {
   if (!sv || !req || !ticket) return eqf_internal_error(EQF_EINVAL, "NULL argument");
-  if (req->params.gs_mode != EQF_GS_WAVEFRONT)
-    return eqf_internal_error(EQF_EINVAL, "the server batches wavefront-mode scenes");
   if (req->nframes < 0) return eqf_internal_error(EQF_EINVAL, "nframes < 0");
   if (req->n_rects < 0 || (req->n_rects > 0 && !req->rects)) return eqf_internal_error(EQF_EINVAL, "bad rects");
   int rc = eqf_internal_check_params(&req->params);
@@ -333,11 +386,19 @@ extern "C" int eqf_server_submit(eqf_server* sv, const eqf_request* req, uint64_
   auto cp = [n](const float* a, std::vector<float>& v) {
     if (a) v.assign(a, a + n);
   };
-  cp(req->src_density, r->sd);
-  cp(req->src_vx, r->su);
-  cp(req->src_vy, r->sv);
-  cp(req->init_vx, r->iu);
-  cp(req->init_vy, r->iv);
+  if (req->borrow_inputs) {
+    r->bsd = req->src_density;
+    r->bsu = req->src_vx;
+    r->bsv = req->src_vy;
+    r->biu = req->init_vx;
+    r->biv = req->init_vy;
+  } else {
+    cp(req->src_density, r->sd);
+    cp(req->src_vx, r->su);
+    cp(req->src_vy, r->sv);
+    cp(req->init_vx, r->iu);
+    cp(req->init_vy, r->iv);
+  }
   r->bytes = estimate_bytes(r->p);
   std::lock_guard<std::mutex> lk(sv->mu);
   if (r->bytes > sv->cfg.hbm_budget_bytes)
@@ -449,9 +510,11 @@ extern "C" int eqf_server_destroy(eqf_server* sv) {
   }
   if (sv->th.joinable()) sv->th.join();
   (void)hipStreamSynchronize(sv->stream);
+  for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
   (void)hipFree(sv->d_descs);
   (void)hipStreamDestroy(sv->stream);
   (void)hipStreamDestroy(sv->astream);
+  for (hipStream_t q : sv->rstream) (void)hipStreamDestroy(q);
   delete sv;
   return EQF_OK;
 }

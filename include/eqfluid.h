@@ -146,8 +146,10 @@ int eqf_batch_last_ms(const eqf_batch *b, double *ms);
  * reference, whose frame loop runs one scene; renderer_helpers.rs:52-72).
  * A server thread admits queued requests while their device memory fits the
  * budget, keeps admitted scenes resident, and advances every resident scene
- * by one frame per tick with one fused launch; scenes join and leave at tick
- * boundaries.  Admission uploads overlap the running tick. */
+ * by one frame per tick: wavefront-mode scenes (size <= 1026) with one fused
+ * launch, red-black scenes (any size) with their frame graphs on a second
+ * stream beside it; scenes join and leave at tick boundaries.  Admission
+ * uploads overlap the running tick. */
 typedef struct eqf_server_config {
     uint64_t hbm_budget_bytes; /* device memory the resident scenes may use */
     int32_t max_active;        /* resident scenes at most (0: no limit) */
@@ -155,10 +157,14 @@ typedef struct eqf_server_config {
 } eqf_server_config;
 
 typedef struct eqf_request {
-    eqf_params params;         /* size, delta_t, diffusion, viscosity, gs_iters; gs_mode must be wavefront */
+    eqf_params params;         /* size, delta_t, diffusion, viscosity, gs_iters, gs_mode */
     int64_t nframes;           /* frames to advance */
     int32_t n_rects;           /* obstacles: n_rects x (x0, y0, x1, y1), fill_obstacle semantics */
     int32_t keep_velocity;     /* 1: also keep vx, vy for eqf_server_result */
+    int32_t borrow_inputs;     /* 1: the arrays below stay valid until the request leaves the
+                                  queue (state != EQF_REQ_QUEUED) and are read in place;
+                                  0: eqf_server_submit copies them */
+    int32_t reserved;
     const int64_t *rects;
     const float *src_density;  /* per-frame sources (size*size, row-major) or NULL */
     const float *src_vx;

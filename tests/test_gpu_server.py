@@ -95,3 +95,33 @@ def test_server_rejects_oversized_and_cancels():
     for sc, t in zip(mix[:4], ts):
         assert_bit_equal(srv.result(t), _solo(sc)[0], f"cancel N={sc['size'] - 2}")
     srv.close()
+
+
+def test_server_red_black_requests_beside_wavefront():
+    """Red-black requests (one past the wavefront's size limit) are served on
+    their own stream beside the fused launch; each result equals its solo
+    red-black run bit for bit, and a small one equals the red-black oracle."""
+    import oracle
+
+    _case("server red-black mix")
+    wf = _mix()[:4]
+    rbs = [scenes.stam_scene(N, scenes.derive_seed(31, N), iters=6, frames=fr, n_rects=2)
+           for N, fr in ((150, 4), (1100, 2), (300, 3))]
+    srv = _abi.Server(hbm_budget_bytes=8 << 30)
+    tw = [srv.submit(sc) for sc in wf]
+    tr = [srv.submit(sc, gs_mode=_abi.GS_REDBLACK) for sc in rbs]
+    srv.wait_idle(120000)
+    for sc, t in zip(wf, tw):
+        assert srv.poll(t).state == _abi.REQ_DONE
+        assert_bit_equal(srv.result(t), _solo(sc)[0], f"wavefront N={sc['size'] - 2}")
+    for sc, t in zip(rbs, tr):
+        st = srv.poll(t)
+        assert st.state == _abi.REQ_DONE and st.frames_done == sc["frames"]
+        s = gpu_scene(sc, _abi.GS_REDBLACK)
+        s.step(sc["frames"])
+        s.sync()
+        assert_bit_equal(srv.result(t), s.download(_abi.DENSITY), f"red-black N={sc['size'] - 2}")
+        s.close()
+    assert_bit_equal(srv.result(tr[0]), run_oracle(rbs[0], order=oracle.RB).f["density"], "red-black vs oracle")
+    assert srv.stats()["scene_frames"] == sum(sc["frames"] for sc in wf + rbs)
+    srv.close()
