@@ -24,6 +24,7 @@ struct eqf_scene {
   hipStream_t stream = nullptr;
   float* d_fld = nullptr;
   float* d_src = nullptr;
+  int src_mask = 0;           // frame sources set: bit 0 density, 1 vx, 2 vy
   uint8_t* d_code = nullptr;
   uint8_t* d_rowf = nullptr;
   int2* d_bnd = nullptr;

@@ -77,7 +77,7 @@ struct SceneDesc {
   float dt0;             // dt * (S-2)         (fluid.rs:390)
   float sf;              // (float)S           (fluid.rs:345, 396)
   float sfm1;            // (float)S - 1.0f    (fluid.rs:403)
-  int has_src;
+  int has_src;           // frame sources to add: bit 0 density (and scratch), 1 vx, 2 vy
   int prio;              // wave issue priority 0..3 in a batch launch (largest scenes highest)
 };
 

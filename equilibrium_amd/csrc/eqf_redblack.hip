@@ -311,19 +311,32 @@ __device__ __forceinline__ float gather(const float* __restrict__ d0, const Trac
 }
 
 // advect, pass A: first column of each row whose back-trace reaches the far
-// edge (the `break` of fluid.rs:420-423); N+1 when none.
+// edge (the `break` of fluid.rs:420-423); N+1 when none.  One workgroup per
+// row; each thread tests 8 columns per round with all 16 loads in flight.
 __global__ void __launch_bounds__(256) k_adv_trig(const SceneDesc* __restrict__ dp, const float* __restrict__ u,
                                                   const float* __restrict__ v, int* __restrict__ trig) {
   const SceneDesc& d = *dp;
   const int S = d.S, N = d.N;
   const int j = blockIdx.x + 1;
+  const int64_t row = (int64_t)j * S;
+  constexpr int K = 8;
   int t = N + 1;
-  for (int i = threadIdx.x + 1; i <= N; i += 256) {
-    const Trace r = back_trace(d, i, j, u[j * S + i], v[j * S + i]);
-    if (r.trig) {
-      t = i;
-      break;
+  for (int base = 1; base <= N; base += 256 * K) {
+    float uu[K], vv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = min(base + k * 256 + (int)threadIdx.x, N);  // in-row address; tested below
+      uu[k] = u[row + i];
+      vv[k] = v[row + i];
     }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = base + k * 256 + threadIdx.x;
+      if (i <= N && back_trace(d, i, j, uu[k], vv[k]).trig) t = min(t, i);
+    }
+    // later rounds hold only larger columns: the block stops together after
+    // the first round in which any column triggered
+    if (__syncthreads_or(t <= N)) break;
   }
   __shared__ int red[256];
   red[threadIdx.x] = t;
@@ -388,11 +401,14 @@ __global__ void k_sources(const SceneDesc* __restrict__ dp) {
   const int64_t n = d.fstride;
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)d.S * d.S) return;
-  const float a = d.src[e];
-  d.fld[F_D * n + e] += a;
-  d.fld[F_SC * n + e] += a;
-  d.fld[F_U * n + e] += d.src[n + e];
-  d.fld[F_V * n + e] += d.src[2 * n + e];
+  const int m = d.has_src;  // only the sources that were set (adding 0 would turn -0 into +0)
+  if (m & 1) {
+    const float a = d.src[e];
+    d.fld[F_D * n + e] += a;
+    d.fld[F_SC * n + e] += a;
+  }
+  if (m & 2) d.fld[F_U * n + e] += d.src[n + e];
+  if (m & 4) d.fld[F_V * n + e] += d.src[2 * n + e];
 }
 
 __global__ void k_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {

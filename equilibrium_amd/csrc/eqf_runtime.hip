@@ -141,7 +141,7 @@ static void fill_desc(eqf_scene* s) {
   d.dt0 = p.delta_t * (float)(s->S - 2);  // fluid.rs:390
   d.sf = (float)s->S;
   d.sfm1 = d.sf - 1.0f;
-  d.has_src = s->d_src != nullptr;
+  d.has_src = s->d_src ? s->src_mask : 0;
   d.prio = 0;
 }
 
@@ -303,6 +303,7 @@ extern "C" int eqf_scene_clone(const eqf_scene* src, eqf_scene** out) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(s->d_fld, src->d_fld, sizeof(float) * NFIELDS * s->fstride, hipMemcpyDeviceToDevice,
                        s->stream);
+  s->src_mask = src->src_mask;
   if (e == hipSuccess && src->d_src) {
     e = hipMalloc(&s->d_src, sizeof(float) * 3 * s->fstride);
     if (e == hipSuccess)
@@ -452,9 +453,12 @@ extern "C" int eqf_set_frame_source(eqf_scene* s, int field, const float* amount
   if (!amounts) {
     HIPCHK(hipMemsetAsync(dst, 0, sizeof(float) * s->fstride, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    s->src_mask &= ~(1 << slot);
     return EQF_OK;
   }
-  return eqf_dev_upload(s, dst, nullptr, amounts, 0);
+  const int rc = eqf_dev_upload(s, dst, nullptr, amounts, 0);
+  if (!rc) s->src_mask |= 1 << slot;
+  return rc;
 }
 
 extern "C" int eqf_upload(eqf_scene* s, int field, const float* host) {

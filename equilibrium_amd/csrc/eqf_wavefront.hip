@@ -598,12 +598,15 @@ __device__ __forceinline__ void add_sources_phase(const SceneDesc& sc, gf* f) {
   gf4* fs = (gf4*)f + F_SC * n4;
   gf4* fu = (gf4*)f + F_U * n4;
   gf4* fv = (gf4*)f + F_V * n4;
+  const int m = sc.has_src;  // only the sources that were set (adding 0 would turn -0 into +0)
   for (int e = threadIdx.x; e < n4; e += blockDim.x) {
-    const f4 a = sd[e], b = su[e], c = sv[e];
-    fd[e] = fd[e] + a;
-    fs[e] = fs[e] + a;
-    fu[e] = fu[e] + b;
-    fv[e] = fv[e] + c;
+    if (m & 1) {
+      const f4 a = sd[e];
+      fd[e] = fd[e] + a;
+      fs[e] = fs[e] + a;
+    }
+    if (m & 2) fu[e] = fu[e] + su[e];
+    if (m & 4) fv[e] = fv[e] + sv[e];
   }
   __syncthreads();
 }
