@@ -451,6 +451,8 @@ static void launch_band_t(hipStream_t st, int T, const SceneDesc* dd, const floa
   if (T == 1) launch_band<MODE, 1, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
   else if (T == 2) launch_band<MODE, 2, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
   else if (T == 4) launch_band<MODE, 4, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else if (T == 5) launch_band<MODE, 5, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+  else if (T == 6) launch_band<MODE, 6, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
   else launch_band<MODE, 8, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
 }
 
@@ -568,7 +570,7 @@ void rb_destroy(void* p) {
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {
   auto* w = static_cast<RBWork*>(s->rb);
   if (!w) return EQF_EINVAL;
-  if (tile < 0 || tile > 2 || (tsweeps != 1 && tsweeps != 2 && tsweeps != 4 && tsweeps != 8)) return EQF_EINVAL;
+  if (tile < 0 || tile > 2 || tsweeps < 1 || tsweeps > 8 || tsweeps == 3 || tsweeps == 7) return EQF_EINVAL;
   if (w->tile != tile || w->tsweeps != tsweeps) {
     w->tile = tile;
     w->tsweeps = tsweeps;

@@ -21,7 +21,7 @@ def _case(label):
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2])
-@pytest.mark.parametrize("tsw", [1, 2, 4, 8])
+@pytest.mark.parametrize("tsw", [1, 2, 4, 5, 6, 8])
 def test_redblack_bit_exact_all_tilings(tile, tsw):
     N = 150
     _case(f"rb tile={tile} T={tsw} N={N}")
