@@ -138,10 +138,12 @@ def cpu_baseline(scs, seconds, threads):
     return cells / wall, wall, frames
 
 
-def bench_redblack_4096(steps, check_frames, N=4096):
+def bench_redblack_4096(steps, check_frames, N=4096, tiling=None):
     """One N=4096 grid, 40 sweeps, red-black mode with register temporal
-    blocking; autotuned band shape; accuracy vs the f64 red-black oracle on the
-    first frames; pressure Poisson residual of the last projection."""
+    blocking; autotuned band shape (or `tiling` = (band, sweeps per launch),
+    e.g. for profiler runs, where the trial frames time differently);
+    accuracy vs the f64 red-black oracle on the first frames; pressure
+    Poisson residual of the last projection."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from equilibrium_amd import tune
 
@@ -151,11 +153,15 @@ def bench_redblack_4096(steps, check_frames, N=4096):
     scenes.apply_to_scene(h, sc)
     h.sync()
     log(f"rb N={N}: scene ready in {time.perf_counter() - t_setup:.1f} s")
-    probe = h.clone()
-    tile, tsw, timings = tune.tune_redblack(probe, frames=1)
-    probe.close()
+    if tiling:
+        tile, tsw = tiling
+        log(f"rb N={N}: band {tune.TILES[tile]} x {tsw} sweeps/launch (given)")
+    else:
+        probe = h.clone()
+        tile, tsw, timings = tune.tune_redblack(probe, frames=1)
+        probe.close()
+        log(f"rb N={N}: autotuned band {tune.TILES[tile]} x {tsw} sweeps/launch: {timings}")
     h.set_tuning(tile, tsw)
-    log(f"rb N={N}: autotuned band {tune.TILES[tile]} x {tsw} sweeps/launch: {timings}")
     h.step(1)
     h.sync()
     t0 = time.perf_counter()
@@ -332,6 +338,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rb-frames", type=int, default=10)
     ap.add_argument("--rb-check-frames", type=int, default=2)
+    ap.add_argument("--rb-tiling", type=str, default="",
+                    help="band:sweeps for the N=4096 red-black grid instead of autotuning (e.g. 0:5)")
     ap.add_argument("--snap-frames", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rates", type=str, default="50,500,2000")
@@ -438,7 +446,8 @@ def main():
 
     if "rb4096" in legs:
         # N=4096 single grid, red-black mode
-        out["n4096_single_redblack"] = bench_redblack_4096(args.rb_frames, args.rb_check_frames)
+        tiling = tuple(int(x) for x in args.rb_tiling.split(":")) if args.rb_tiling else None
+        out["n4096_single_redblack"] = bench_redblack_4096(args.rb_frames, args.rb_check_frames, tiling=tiling)
         log(f"N=4096 red-black: {out['n4096_single_redblack']['value']:.4g} cell-updates/s")
 
     if "snap" in legs:

@@ -196,6 +196,10 @@ extern "C" int eqf_snapshot_capture(eqf_snapshot* sn, int64_t frame) {
   if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, hipGetErrorString(e));
   auto& sl = sn->slots[(sn->head + sn->count) % sn->depth];
   const size_t n = (size_t)s->S * s->S;
+  // A slot released without an acquire may still be copying out: its next
+  // contents wait for that copy.  (An event never recorded is complete.)
+  e = hipStreamWaitEvent(s->stream, sl.done, 0);
+  if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, std::string("snapshot: ") + hipGetErrorString(e));
   int rc = render_inputs(s, sl.d_dens);
   if (!rc && sn->with_rgba) rc = launch_render(s, sl.d_dens, &sn->colors, sl.d_rgba);
   if (rc) return rc;
