@@ -91,3 +91,26 @@ def test_rectangles_do_not_overlap_and_stay_inside():
         for a_ in range(len(sc["rects"])):
             for b_ in range(a_ + 1, len(sc["rects"])):
                 assert not scenes._overlap(sc["rects"][a_], sc["rects"][b_])
+
+
+def test_default_colours_mirror_reference():
+    """FluidConfigs::default colours (configs.rs:56-57) and the renderer's
+    obstacle colour, egui's Color32::RED (renderer_helpers.rs:101)."""
+    c = _abi.make_colors()
+    assert list(c.fluid) == [208, 88, 157, 220]
+    assert list(c.world) == [94, 146, 162, 128]
+    assert list(c.obstacle) == [255, 0, 0, 255]
+    fc = FluidConfigs.default()
+    assert tuple(fc.fluid_color) == tuple(c.fluid) and tuple(fc.world_color) == tuple(c.world)
+    c2 = _abi.make_colors(fluid=(1, 2, 3, 4))
+    assert list(c2.fluid) == [1, 2, 3, 4] and list(c2.world) == [94, 146, 162, 128]
+
+
+def test_render_and_snapshot_reject_null_arguments():
+    L = _abi.lib()
+    assert L.eqf_render_rgba(None, None, None) == _abi.EQF_EINVAL
+    assert L.eqf_snapshot_create(None, 2, 0, None, None) == _abi.EQF_EINVAL
+    assert L.eqf_snapshot_capture(None, 0) == _abi.EQF_EINVAL
+    assert L.eqf_snapshot_acquire(None, None, None, None) == _abi.EQF_EINVAL
+    assert L.eqf_snapshot_release(None) == _abi.EQF_EINVAL
+    assert L.eqf_snapshot_destroy(None) == _abi.EQF_OK
