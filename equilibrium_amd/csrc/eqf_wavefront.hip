@@ -179,6 +179,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int lane = threadIdx.x & 63;
   const int wave = uni(threadIdx.x >> 6);
   const int RO = ns * P * 64;
+  const int HP = cyc_lds_row(P);  // H row stride (floats)
   gf* __restrict__ ring = X + RO;
   // Sweep streams: with m waves per strip, stream j of a strip runs sweeps
   // j, j+m, j+2m, ... back to back, LAG steps behind stream j-1 (stream 0
@@ -222,14 +223,14 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
     for (int t = lane; t < P; t += 64) {
       int c = t - 63;  // column - 1 of the reading lane 63 of the strip above at its slot t
       c += c < 0 ? P : 0;
-      H[(ns - 1 + s) * P + t] = c < N ? X[(s * P + c) * 64] : 0.0f;
+      H[(ns - 1 + s) * HP + t] = c < N ? X[(s * P + c) * 64] : 0.0f;
     }
   if (has_strip && jst == 0 && s == ns - 1)
     for (int t = lane; t < P; t += 64) {
       int c = t - rl;
       c += c < 0 ? P : 0;
       if (rl == 63) {
-        H[(2 * ns - 1) * P + t] = c < N ? ring[S + c + 1] : 0.0f;
+        H[(2 * ns - 1) * HP + t] = c < N ? ring[S + c + 1] : 0.0f;
       } else {  // row N+1 at lane rl+1: slot (c - 1 + rl + 1) mod P holds column c
         int cm = c - 1;
         cm += cm < 0 ? P : 0;
@@ -254,10 +255,10 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const bool bot = last && !has_next;                 // down neighbour is the bottom ring
   const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
   const int dn_row = has_next ? ns + s : 2 * ns - 1;  // post row of the strip below, or the bottom ring
-  const int e_row = (last_e ? dn_row : up_row) * P;  // H row of this lane's edge value (floats)
-  const int TR = 2 * ns * P + lane * 33;              // this lane's scratch slot in LDS
-  const int raw_row = (1 + s) * P;                    // this strip's raw last row (if has_next)
-  const int post_row = (ns - 1 + s) * P;              // this strip's post row 0 (if has_prev)
+  const int e_row = (last_e ? dn_row : up_row) * HP;  // H row of this lane's edge value (floats)
+  const int TR = 2 * ns * HP + lane * 33;              // this lane's scratch slot in LDS
+  const int raw_row = (1 + s) * HP;                    // this strip's raw last row (if has_next)
+  const int post_row = (ns - 1 + s) * HP;              // this strip's post row 0 (if has_prev)
   const bool raw_ok = last && valid && has_next;      // lane 63 hands its raw values down
   const bool post_ok = first && has_prev;             // lane 0 hands its final values up
   const int sb = uni(s * P * 256);                        // strip base in X (bytes)
