@@ -95,12 +95,12 @@ EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
 
 // Exact mode keeps the strips' hand-off rows in LDS: 2*ns rows of P floats
 // (top ring, raw last rows, post-boundary first rows, bottom ring) plus a
-// scratch slot per lane that absorbs the stores of lanes with nothing to hand
-// off (33 floats apart, so the 64 lanes' stores fall in distinct banks).
+// 16-float sink after them that absorbs the stores of lanes with nothing to
+// hand off (all such lanes of a store write the same float: no bank conflict).
 // Rows are P + 1 floats apart, so that lane 63's edge read (the strip below)
 // and the other lanes' shared read (the strip above) fall in different banks.
 EQF_HD inline int cyc_lds_row(int P) { return P + 1; }
-EQF_HD inline int cyc_lds_floats(int ns, int P) { return 2 * ns * cyc_lds_row(P) + 64 * 33; }
+EQF_HD inline int cyc_lds_floats(int ns, int P) { return 2 * ns * cyc_lds_row(P) + 16; }
 
 // Ring offset inside a cyclic field.
 EQF_HD inline int64_t cyc_ring(int P, int ns) { return (int64_t)ns * P * 64; }

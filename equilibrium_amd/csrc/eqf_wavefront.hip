@@ -256,7 +256,10 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
   const int dn_row = has_next ? ns + s : 2 * ns - 1;  // post row of the strip below, or the bottom ring
   const int e_row = (last_e ? dn_row : up_row) * HP;  // H row of this lane's edge value (floats)
-  const int TR = 2 * ns * HP + lane * 33;              // this lane's scratch slot in LDS
+  // Sink for the hand-off stores of lanes with nothing to hand off: one
+  // float shared by every lane (same-address writes of a wave do not
+  // conflict; no lane reads it).
+  const int TR = 2 * ns * HP;
   const int raw_row = (1 + s) * HP;                    // this strip's raw last row (if has_next)
   const int post_row = (ns - 1 + s) * HP;              // this strip's post row 0 (if has_prev)
   const bool raw_ok = last && valid && has_next;      // lane 63 hands its raw values down
