@@ -89,7 +89,9 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict
   constexpr int H = 3 * T, W = 128, RH = NW * R;
   constexpr int TX = W - 2 * H, TY = RH - 2 * H;
   static_assert(TX > 0 && TY > 0 && TX % 2 == 0 && TY % 2 == 0 && R % 2 == 0, "band geometry");
-  __shared__ float2 edg[2][2][NW][64];  // [buffer][first / last row][wave][lane]
+  // [buffer][first / last row][column of the lane's pair][wave][lane]: one
+  // float per lane and store, so a wave's 64 accesses hit 64 distinct banks
+  __shared__ float edg[2][2][2][NW][64];
   const SceneDesc& d = *dp;
   const int S = d.S, N = d.N;
   const int lane = threadIdx.x & 63;
@@ -180,11 +182,13 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict
       for (int i = 0; i < R; ++i) asm volatile("" : "+v"(cd[i]));
       float2 ab = make_float2(0.0f, 0.0f), be = make_float2(0.0f, 0.0f);  // rows above / below the band
       if (ph < 2 || MODE != M_ROW) {
-        edg[par][0][w][lane] = make_float2(x[0][0], x[0][1]);
-        edg[par][1][w][lane] = make_float2(x[R - 1][0], x[R - 1][1]);
+        edg[par][0][0][w][lane] = x[0][0];
+        edg[par][0][1][w][lane] = x[0][1];
+        edg[par][1][0][w][lane] = x[R - 1][0];
+        edg[par][1][1][w][lane] = x[R - 1][1];
         __syncthreads();
-        if (w > 0) ab = edg[par][1][w - 1][lane];
-        if (w < NW - 1) be = edg[par][0][w + 1][lane];
+        if (w > 0) ab = make_float2(edg[par][1][0][w - 1][lane], edg[par][1][1][w - 1][lane]);
+        if (w < NW - 1) be = make_float2(edg[par][0][0][w + 1][lane], edg[par][0][1][w + 1][lane]);
         par ^= 1;
       }
       if (ph < 2) {
