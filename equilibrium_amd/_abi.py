@@ -122,6 +122,7 @@ def lib():
             "eqf_get_mask": (I, [P, P]),
             "eqf_add_density": (I, [P, U32, U32, F]),
             "eqf_add_velocity": (I, [P, U32, U32, F, F]),
+            "eqf_add_noise": (I, [P, F, U32, U32]),
             "eqf_add_source": (I, [P, I, P]),
             "eqf_set_frame_source": (I, [P, I, P]),
             "eqf_upload": (I, [P, I, P]),
@@ -257,6 +258,9 @@ class Scene:
 
     def add_velocity(self, x, y, ax, ay):
         check(lib().eqf_add_velocity(self.h, x, y, ax, ay))
+
+    def add_noise(self, angle_degrees, rand_x, rand_y):
+        check(lib().eqf_add_noise(self.h, angle_degrees, rand_x, rand_y))
 
     def add_source(self, field, amounts):
         a = np.ascontiguousarray(amounts, np.float32).ravel()

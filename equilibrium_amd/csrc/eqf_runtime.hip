@@ -431,6 +431,17 @@ extern "C" int eqf_add_velocity(eqf_scene* s, uint32_t x, uint32_t y, float ax, 
   return EQF_OK;
 }
 
+extern "C" int eqf_add_noise(eqf_scene* s, float angle_degrees, uint32_t rand_x, uint32_t rand_y) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  // rotate (rand_x, rand_y) about the centre by angle_degrees, in f32
+  const float c = (float)(s->S / 2);
+  const double th = (double)angle_degrees * 3.14159265358979323846 / 180.0;
+  const float cs = (float)std::cos(th), sn = (float)std::sin(th);
+  const float dx = (float)rand_x - c, dy = (float)rand_y - c;
+  const float rx = c + (dx * cs - dy * sn), ry = c + (dx * sn + dy * cs);
+  return eqf_add_velocity(s, (uint32_t)(s->S / 2), (uint32_t)(s->S / 2), rx * 2.0f, ry * 2.0f);
+}
+
 extern "C" int eqf_add_source(eqf_scene* s, int field, const float* amounts) {
   if (!s || !amounts) return fail(EQF_EINVAL, "NULL argument");
   if (field != EQF_DENSITY && field != EQF_VX && field != EQF_VY)

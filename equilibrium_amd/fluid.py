@@ -199,17 +199,18 @@ class Fluid:
     # [dt, dt] and a point from rand's thread_rng, rotates the point about the
     # centre with geo-0.18 and adds twice its coordinates to the centre cell's
     # velocity.  Those crates are not available and thread_rng is unseeded, so
-    # this takes the angle (degrees, as geo's rotate) and the point from the
-    # caller: parity unpinned.
-    def add_noise(self, angle_degrees, rand_x, rand_y):
+    # the angle (degrees, as geo's rotate) comes from the caller and the point
+    # either from the caller or from `rng`, a seeded stand-in for thread_rng
+    # (an object with randint(lo, hi), e.g. scenes.SplitMix64): parity
+    # unpinned.  The rotation and the add run in the library (eqf_add_noise).
+    def add_noise(self, angle_degrees, rand_x=None, rand_y=None, rng=None):
         size = self.simulation_configs.size
-        cx = float(size // 2)
-        cy = float(size // 2)
-        th = math.radians(angle_degrees)
-        dx, dy = float(rand_x) - cx, float(rand_y) - cy
-        rx = np.float32(cx + dx * math.cos(th) - dy * math.sin(th))
-        ry = np.float32(cy + dx * math.sin(th) + dy * math.cos(th))
-        self.add_velocity(int(cx), int(cy), float(rx * np.float32(2.0)), float(ry * np.float32(2.0)))
+        if rand_x is None or rand_y is None:
+            if rng is None:
+                raise ValueError("add_noise needs the random point or a seeded rng")
+            rand_x = rng.randint(0, size - 1)  # gen_range(0..size)
+            rand_y = rng.randint(0, size - 1)
+        self._scene.add_noise(angle_degrees, int(rand_x), int(rand_y))
 
 
 class RenderingListener:

@@ -124,6 +124,12 @@ class Fluid {
     detail::check(eqf_add_velocity(scene_.get(), x, y, ax, ay));
   }
 
+  // fluid.rs:575-599 with the Perlin angle and the thread_rng point supplied
+  // by the caller (the noise/rand/geo crates are not reproduced)
+  void add_noise(float angle_degrees, uint32_t rand_x, uint32_t rand_y) {
+    detail::check(eqf_add_noise(scene_.get(), angle_degrees, rand_x, rand_y));
+  }
+
   // fluid.rs:437-524
   void step() {
     sync_params();

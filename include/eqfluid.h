@@ -107,6 +107,13 @@ int eqf_get_mask(const eqf_scene *s, uint8_t *mask);
  * clamped index; density is added to the density and scratch fields. */
 int eqf_add_density(eqf_scene *s, uint32_t x, uint32_t y, float amount);
 int eqf_add_velocity(eqf_scene *s, uint32_t x, uint32_t y, float ax, float ay);
+/* Fluid::add_noise (fluid.rs:575-599) with its random inputs supplied by the
+ * caller: the line from the centre cell (size/2, size/2) to (rand_x, rand_y)
+ * is rotated by `angle_degrees` about the centre, and twice the rotated end
+ * point is added to the centre cell's velocity.  The reference draws the
+ * angle from noise-0.7 Perlin at [dt, dt] and the point from an unseeded
+ * thread_rng; neither is reproduced (its parity is unpinned). */
+int eqf_add_noise(eqf_scene *s, float angle_degrees, uint32_t rand_x, uint32_t rand_y);
 /* add_density(x, y, a[idx]) for every cell (field EQF_DENSITY), or
  * add_velocity with one component (EQF_VX / EQF_VY). */
 int eqf_add_source(eqf_scene *s, int field, const float *amounts);

@@ -64,11 +64,11 @@ def test_null_and_invalid_arguments_fail_cleanly():
 
 
 def test_ctx_create_without_gpu_reports_device_error():
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
     h = ctypes.c_void_p()
     rc = _abi.lib().eqf_ctx_create(0, ctypes.byref(h))
+    if rc == _abi.EQF_OK:  # a GPU is present: nothing to check here
+        _abi.lib().eqf_ctx_destroy(h)
+        pytest.skip("a GPU is present")
     assert rc in (_abi.EQF_EDEVICE, _abi.EQF_EINVAL)
     assert _abi.last_error()
 

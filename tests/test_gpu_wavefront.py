@@ -192,3 +192,48 @@ def test_batch_mixed_sizes_equals_solo():
     for sc, h in zip(scs, handles):
         st = run_oracle(sc)
         assert_state_equal(download_all(h), st, f"batch N={sc['size'] - 2}")
+
+
+def test_add_noise_rotation_closed_forms():
+    """add_noise (fluid.rs:575-599) with the random inputs supplied: at 0, 90
+    and 180 degrees the rotated end point has a closed form (the centre is
+    (size/2, size/2)); twice it is added to the centre cell's velocity."""
+    _case("add_noise")
+    size = 40
+    c = size // 2
+    for ang, (rx, ry) in ((0.0, (31, 7)), (90.0, (31, 7)), (180.0, (12, 25))):
+        s = _abi.Scene(_noise_params(size))
+        u0, v0 = s.download(_abi.VX), s.download(_abi.VY)
+        s.add_noise(ang, rx, ry)
+        s.sync()
+        du = s.download(_abi.VX) - u0
+        dv = s.download(_abi.VY) - v0
+        dx, dy = rx - c, ry - c
+        want = {0.0: (rx, ry), 90.0: (c - dy, c + dx), 180.0: (c - dx, c - dy)}[ang]
+        k = c + c * size
+        assert np.count_nonzero(du) <= 1 and np.count_nonzero(dv) <= 1
+        assert du[k] == pytest.approx(2.0 * want[0], abs=1e-4), (ang, du[k], want)
+        assert dv[k] == pytest.approx(2.0 * want[1], abs=1e-4), (ang, dv[k], want)
+        s.close()
+
+
+def test_add_noise_seeded_point_is_deterministic():
+    from equilibrium_amd.fluid import Fluid, FluidConfigs, SimulationConfigs
+
+    _case("add_noise seeded")
+    outs = []
+    for _ in range(2):
+        f = Fluid(FluidConfigs(), SimulationConfigs(0.1, 4, 50))
+        rng = scenes.SplitMix64(1234)
+        for _ in range(3):
+            f.add_noise(37.0, rng=rng)
+            f.step()
+        outs.append((f.velocities_x, f.velocities_y))
+    assert_bit_equal(outs[0][0], outs[1][0], "vx")
+    assert_bit_equal(outs[0][1], outs[1][1], "vy")
+
+
+def _noise_params(size):
+    p = _abi.default_params()
+    p.size = size
+    return p
