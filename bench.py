@@ -74,8 +74,13 @@ def make_scenes(scs):
 
 
 def bench_batch(scs, steps, warmup):
+    from equilibrium_amd import tune
+
     hs = make_scenes(scs)
+    w, o, timings = tune.tune_batch(hs)
+    log(f"batch: autotuned max waves {w}, order {o}: {timings}")
     b = _abi.Batch(hs)
+    b.set_tuning(w, o)
     if warmup:
         b.step(warmup)
         b.sync()
@@ -89,7 +94,7 @@ def bench_batch(scs, steps, warmup):
     b.close()
     for h in hs:
         h.close()
-    return wall, ev_ms
+    return wall, ev_ms, {"max_waves": w, "order": "largest first" if o == 0 else "given", "trial_ms": timings}
 
 
 def bench_single_dropin(scs, frames):
@@ -369,7 +374,7 @@ def main():
         iters = mix[0]["iters"]
         cells = sum((sc["size"] - 2) ** 2 for sc in mix)
         log(f"batched mix: {len(mix)} scenes, {cells} interior cells, {iters} sweeps")
-        wall, ev_ms = bench_batch(mix, args.steps, args.warmup)
+        wall, ev_ms, tuning = bench_batch(mix, args.steps, args.warmup)
         value = cells * args.steps / wall
         ms_per_step = wall * 1e3 / args.steps
         launch_ms = ev_ms / args.steps  # HIP events on the batch stream around the K launches
@@ -398,6 +403,7 @@ def main():
                 "interior_cells": cells,
                 "gs_sweeps": iters,
                 "mode": "wavefront (bit-exact vs CPU oracle)",
+                "batch_tuning": tuning,
             },
             "roofline": {
                 "bound": "hbm",

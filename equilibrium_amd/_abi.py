@@ -18,6 +18,7 @@ EQF_OK, EQF_EINVAL, EQF_ENOMEM, EQF_EDEVICE, EQF_EBUSY, EQF_ENOTFOUND = range(6)
 DENSITY, SCRATCH, VX, VY, VX0, VY0 = range(6)
 DEFAULT_WALL, NO_WALL = 0, 1
 GS_WAVEFRONT, GS_REDBLACK = 0, 1
+BATCH_LARGEST_FIRST, BATCH_GIVEN_ORDER = 0, 1
 
 
 class EqfError(RuntimeError):
@@ -135,6 +136,7 @@ def lib():
             "eqf_batch_sync": (I, [P]),
             "eqf_batch_destroy": (I, [P]),
             "eqf_batch_last_ms": (I, [P, ctypes.POINTER(ctypes.c_double)]),
+            "eqf_batch_set_tuning": (I, [P, I, I]),
             "eqf_server_create": (I, [P, ctypes.POINTER(ServerConfig), PP]),
             "eqf_server_submit": (I, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
             "eqf_server_poll": (I, [P, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]),
@@ -361,6 +363,9 @@ class Batch:
         ms = ctypes.c_double()
         check(lib().eqf_batch_last_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def set_tuning(self, max_waves, order=BATCH_LARGEST_FIRST):
+        check(lib().eqf_batch_set_tuning(self.h, max_waves, order))
 
     def close(self):
         if getattr(self, "h", None):

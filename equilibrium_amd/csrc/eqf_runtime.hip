@@ -532,15 +532,49 @@ extern "C" int eqf_sync(eqf_scene* s) {
 // ------------------------------------------------------------------ batch --
 struct eqf_batch {
   eqf_ctx* ctx = nullptr;
-  std::vector<eqf_scene*> scenes;
+  std::vector<eqf_scene*> given;   // the caller's order
+  std::vector<eqf_scene*> scenes;  // workgroup order of the launch
   SceneDesc* d_descs = nullptr;
   int W = 1;
+  int max_waves = 16;  // tuning: waves per workgroup at most (>= the largest strip count)
+  int order = EQF_BATCH_LARGEST_FIRST;
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   int64_t graph_frames = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
 };
+
+// Workgroup order, descriptors and workgroup width of a batch from its
+// tuning; uploads the descriptors on the batch stream.
+static int batch_layout(eqf_batch* b) {
+  b->scenes = b->given;
+  if (b->order == EQF_BATCH_LARGEST_FIRST) {
+    // their workgroups are the longest: dispatched first (longest-processing-
+    // time order), they spread over the XCDs and set the launch's length
+    std::stable_sort(b->scenes.begin(), b->scenes.end(), [](const eqf_scene* a, const eqf_scene* c) {
+      return (int64_t)a->N * a->N * std::max(a->desc.diff_iters, 1) >
+             (int64_t)c->N * c->N * std::max(c->desc.diff_iters, 1);
+    });
+  }
+  const int n = (int)b->scenes.size();
+  std::vector<SceneDesc> descs(n);
+  int maxns = 1, maxw = 1;
+  for (int i = 0; i < n; ++i) {
+    const eqf_scene* s = b->scenes[i];
+    descs[i] = s->desc;
+    maxns = std::max(maxns, s->ns);
+    maxw = std::max(maxw, eqf_scene_waves(s->ns, s->P));
+  }
+  if (b->max_waves < maxns) return fail(EQF_EINVAL, "max_waves below the largest scene's strip count");
+  for (int i = 0; i < n; ++i) descs[i].prio = eqf_batch_prio(b->scenes[i]->ns, maxns);
+  b->W = std::min(maxw, b->max_waves);
+  hipError_t e =
+      hipMemcpyAsync(b->d_descs, descs.data(), sizeof(SceneDesc) * n, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return fail(EQF_EDEVICE, std::string("batch layout: ") + hipGetErrorString(e));
+  return EQF_OK;
+}
 
 extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, eqf_batch** out) {
   if (!ctx || !scenes || n <= 0 || !out) return fail(EQF_EINVAL, "bad arguments");
@@ -550,44 +584,49 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
     if (scenes[i]->in_batch) return fail(EQF_EBUSY, "scene already in a batch");
   }
   HIPCHK(hipSetDevice(ctx->device));
+  for (int i = 0; i < n; ++i) {
+    int rc = sync_desc(scenes[i]);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(scenes[i]->stream));
+  }
   auto* b = new eqf_batch;
   b->ctx = ctx;
-  // Largest scenes first: their workgroups are the longest, so they are
-  // dispatched first (longest-processing-time order) and spread over XCDs.
-  b->scenes.assign(scenes, scenes + n);
-  std::stable_sort(b->scenes.begin(), b->scenes.end(), [](const eqf_scene* a, const eqf_scene* c) {
-    return (int64_t)a->N * a->N * std::max(a->desc.diff_iters, 1) > (int64_t)c->N * c->N * std::max(c->desc.diff_iters, 1);
-  });
-  std::vector<SceneDesc> descs(n);
-  int maxns = 1, maxw = 1;
-  for (int i = 0; i < n; ++i) {
-    eqf_scene* s = b->scenes[i];
-    int rc = sync_desc(s);
-    if (rc) {
-      delete b;
-      return rc;
-    }
-    HIPCHK(hipStreamSynchronize(s->stream));
-    descs[i] = s->desc;
-    maxns = std::max(maxns, s->ns);
-    maxw = std::max(maxw, eqf_scene_waves(s->ns, s->P));
-  }
-  for (int i = 0; i < n; ++i) descs[i].prio = eqf_batch_prio(b->scenes[i]->ns, maxns);
-  b->W = maxw;
+  b->given.assign(scenes, scenes + n);
   hipError_t e = hipStreamCreate(&b->stream);
   if (e == hipSuccess) e = hipMalloc(&b->d_descs, sizeof(SceneDesc) * n);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_descs, descs.data(), sizeof(SceneDesc) * n, hipMemcpyHostToDevice, b->stream);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-  if (e != hipSuccess) {
+  int rc = e == hipSuccess ? batch_layout(b) : fail(EQF_EDEVICE, std::string("batch create: ") + hipGetErrorString(e));
+  if (rc) {
     (void)hipFree(b->d_descs);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
-    return fail(EQF_EDEVICE, std::string("batch create: ") + hipGetErrorString(e));
+    return rc;
   }
   for (auto* s : b->scenes) s->in_batch = 1;
   *out = b;
+  return EQF_OK;
+}
+
+extern "C" int eqf_batch_set_tuning(eqf_batch* b, int max_waves, int order) {
+  if (!b) return fail(EQF_EINVAL, "batch is NULL");
+  if (max_waves < 1 || max_waves > 16) return fail(EQF_EINVAL, "max_waves must be 1..16");
+  if (order != EQF_BATCH_LARGEST_FIRST && order != EQF_BATCH_GIVEN_ORDER) return fail(EQF_EINVAL, "bad order");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const int mw = b->max_waves, od = b->order;
+  b->max_waves = max_waves;
+  b->order = order;
+  int rc = batch_layout(b);
+  if (rc) {  // keep the previous tuning
+    b->max_waves = mw;
+    b->order = od;
+    (void)batch_layout(b);
+    return rc;
+  }
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);  // the graph holds the launch shape
+  b->gexec = nullptr;
+  b->graph_frames = 0;
   return EQF_OK;
 }
 

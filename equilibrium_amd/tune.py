@@ -1,11 +1,16 @@
-"""Autotuner for the red-black mode: band shape x sweeps per launch.
+"""Autotuners.
 
-For a scene it times one frame per configuration on the GPU and keeps the
-fastest; results are cached per (size, sweeps, device name) in a JSON file
-(EQF_TUNE_CACHE, default equilibrium_amd/tuning_cache.json) so later runs
-reuse them.  The wavefront mode has no tile knobs: its launch shape is one
-workgroup per scene and one wave per 64-row strip.
+* Red-black mode: band shape x sweeps per launch (eqf_scene_set_tuning).
+* Batches (wavefront mode): waves per workgroup at most -- with fewer waves a
+  strip runs fewer sweep streams -- and the scene-to-workgroup order
+  (eqf_batch_set_tuning).
+
+Each times frames per configuration on the GPU (on clones, so the tuned
+scenes are not advanced) and keeps the fastest; results are cached in a JSON
+file (EQF_TUNE_CACHE, default equilibrium_amd/tuning_cache.json) keyed by the
+workload and the device name, so later runs reuse them.
 """
+import hashlib
 import json
 import os
 import time
@@ -72,3 +77,50 @@ def tune_redblack(scene, frames=1, use_cache=True):
     db[key] = {"best": [tile, tsw], "ms": timings, "tiles": {str(k): v for k, v in TILES.items()}}
     _save(db)
     return tile, tsw, timings
+
+
+BATCH_WAVES = (16, 8, 4, 2, 1)
+BATCH_ORDERS = (_abi.BATCH_LARGEST_FIRST, _abi.BATCH_GIVEN_ORDER)
+
+
+def _batch_key(scenes):
+    sig = ",".join(f"{s.params.size}:{s.params.gs_iters}:{s.params.pressure_iters}" for s in scenes)
+    return f"batch:{len(scenes)}:{hashlib.sha1(sig.encode()).hexdigest()[:16]}:{_device_name()}"
+
+
+def tune_batch(scenes, frames=2, use_cache=True):
+    """Picks (max_waves, order) for a batch of the given wavefront-mode
+    _abi.Scene objects and returns (max_waves, order, timings in ms per
+    frame).  Trials run on clones."""
+    key = _batch_key(scenes)
+    db = _load() if use_cache else {}
+    if key in db:
+        w, o = db[key]["best"]
+        return w, o, db[key].get("ms", {})
+    maxns = max((s.params.size - 2 + 63) // 64 for s in scenes)
+    clones = [s.clone() for s in scenes]
+    b = _abi.Batch(clones)
+    timings = {}
+    try:
+        for w in BATCH_WAVES:
+            if w < maxns:
+                continue
+            for o in BATCH_ORDERS:
+                b.set_tuning(w, o)
+                b.step(1)  # builds the graph
+                b.sync()
+                t0 = time.perf_counter()
+                b.step(frames)
+                b.sync()
+                timings[f"{w}:{o}"] = (time.perf_counter() - t0) * 1e3 / frames
+    finally:
+        b.close()
+        for c in clones:
+            c.close()
+    best = min(timings, key=timings.get)
+    w, o = (int(x) for x in best.split(":"))
+    db = _load() if use_cache else {}
+    db[key] = {"best": [w, o], "ms": timings}
+    if use_cache:
+        _save(db)
+    return w, o, timings

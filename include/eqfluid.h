@@ -148,6 +148,13 @@ int eqf_batch_sync(eqf_batch *b);
 int eqf_batch_destroy(eqf_batch *b);
 /* Wall time of the last eqf_batch_step in milliseconds (HIP events). */
 int eqf_batch_last_ms(const eqf_batch *b, double *ms);
+/* Launch shape of a batch (no counterpart in the reference): waves per
+ * workgroup at most (1..16, at least the largest scene's strip count of
+ * ceil(N/64); fewer waves means fewer sweep streams per strip) and the
+ * scene-to-workgroup order.  Results do not depend on it.  Chosen by the
+ * autotuner (equilibrium_amd/tune.py). */
+enum { EQF_BATCH_LARGEST_FIRST = 0, EQF_BATCH_GIVEN_ORDER = 1 };
+int eqf_batch_set_tuning(eqf_batch *b, int max_waves, int order);
 
 /* Serving: continuous batching of scene requests (no counterpart in the
  * reference, whose frame loop runs one scene; renderer_helpers.rs:52-72).

@@ -237,3 +237,27 @@ def _noise_params(size):
     p = _abi.default_params()
     p.size = size
     return p
+
+
+def test_batch_tuning_keeps_results():
+    """Every launch shape the batch autotuner may pick (waves per workgroup,
+    workgroup order) gives the oracle's bits."""
+    _case("batch tuning")
+    scs = [scenes.stam_scene(N, scenes.derive_seed(60, N), iters=5, frames=2, n_rects=2)
+           for N in (40, 130, 200, 70)]
+    want = [run_oracle(sc, frames=2) for sc in scs]
+    for w, o in ((16, 0), (8, 1), (4, 0), (4, 1)):
+        hs = [gpu_scene(sc) for sc in scs]
+        b = _abi.Batch(hs)
+        b.set_tuning(w, o)
+        b.step(2)
+        b.sync()
+        b.close()
+        for sc, h, st in zip(scs, hs, want):
+            assert_state_equal(download_all(h), st, f"tuning {w}:{o} N={sc['size'] - 2}")
+            h.close()
+    hs = [gpu_scene(sc) for sc in scs]
+    b = _abi.Batch(hs)
+    with pytest.raises(_abi.EqfError):
+        b.set_tuning(2, 0)  # N=200 needs 4 strips
+    b.close()
