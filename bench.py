@@ -73,12 +73,16 @@ def make_scenes(scs):
     return hs
 
 
-def bench_batch(scs, steps, warmup):
+def bench_batch(scs, steps, warmup, tuning=None):
     from equilibrium_amd import tune
 
     hs = make_scenes(scs)
-    w, o, timings = tune.tune_batch(hs)
-    log(f"batch: autotuned max waves {w}, order {o}: {timings}")
+    if tuning:
+        (w, o), timings = tuning, {}
+        log(f"batch: max waves {w}, order {o} (given)")
+    else:
+        w, o, timings = tune.tune_batch(hs)
+        log(f"batch: autotuned max waves {w}, order {o}: {timings}")
     b = _abi.Batch(hs)
     b.set_tuning(w, o)
     if warmup:
@@ -343,6 +347,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rb-frames", type=int, default=10)
     ap.add_argument("--rb-check-frames", type=int, default=2)
+    ap.add_argument("--batch-tuning", type=str, default="",
+                    help="max_waves:order for the batch instead of autotuning (e.g. 16:0)")
     ap.add_argument("--rb-tiling", type=str, default="",
                     help="band:sweeps for the N=4096 red-black grid instead of autotuning (e.g. 0:5)")
     ap.add_argument("--snap-frames", type=int, default=20)
@@ -374,7 +380,8 @@ def main():
         iters = mix[0]["iters"]
         cells = sum((sc["size"] - 2) ** 2 for sc in mix)
         log(f"batched mix: {len(mix)} scenes, {cells} interior cells, {iters} sweeps")
-        wall, ev_ms, tuning = bench_batch(mix, args.steps, args.warmup)
+        given = tuple(int(x) for x in args.batch_tuning.split(":")) if args.batch_tuning else None
+        wall, ev_ms, tuning = bench_batch(mix, args.steps, args.warmup, given)
         value = cells * args.steps / wall
         ms_per_step = wall * 1e3 / args.steps
         launch_ms = ev_ms / args.steps  # HIP events on the batch stream around the K launches
