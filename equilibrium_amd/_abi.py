@@ -137,6 +137,7 @@ def lib():
             "eqf_batch_destroy": (I, [P]),
             "eqf_batch_last_ms": (I, [P, ctypes.POINTER(ctypes.c_double)]),
             "eqf_batch_set_tuning": (I, [P, I, I]),
+            "eqf_batch_set_band": (I, [P, I, I]),
             "eqf_server_create": (I, [P, ctypes.POINTER(ServerConfig), PP]),
             "eqf_server_submit": (I, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
             "eqf_server_poll": (I, [P, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]),
@@ -366,6 +367,10 @@ class Batch:
 
     def set_tuning(self, max_waves, order=BATCH_LARGEST_FIRST):
         check(lib().eqf_batch_set_tuning(self.h, max_waves, order))
+
+    def set_band(self, tile, sweeps_per_block):
+        """Band shape and sweeps per launch of a red-black batch."""
+        check(lib().eqf_batch_set_band(self.h, tile, sweeps_per_block))
 
     def close(self):
         if getattr(self, "h", None):

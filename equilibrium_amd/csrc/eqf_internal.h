@@ -72,4 +72,9 @@ void* rb_create(eqf_scene* s, std::string& err);
 void rb_destroy(void* rb);
 int rb_step(eqf_scene* s, int64_t nframes, std::string& err);
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps);
+// A batch of red-black scenes: every frame kernel covers all of them.
+void* rb_batch_create(const std::vector<eqf_scene*>& scenes, hipStream_t st, std::string& err);
+void rb_batch_destroy(void* g);
+int rb_batch_set_tiling(void* g, int tile, int tsweeps);
+int rb_batch_step(void* g, int64_t nframes, std::string& err);
 }  // namespace eqf

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -66,6 +67,40 @@ enum BandSel : uint32_t {
   BS_UPD = 16,
 };
 
+// One scene of a red-black launch group: a single scene, or every scene of
+// a batch.  Each kernel below covers all scenes of its group in one launch;
+// a workgroup finds its scene by binary search over a prefix table of the
+// launch's work units (band tiles, rows, cells or list entries per scene).
+struct RbScene {
+  SceneDesc d;  // the scene's descriptor (a copy): fields, sizes, coefficients, rule bytes, lists
+  float* tmp;   // its ping-pong field
+  int* trig;    // its per-row break columns
+};
+// Field roles in kernel arguments: F_* (eqf_layout.h), the ping-pong field, or none.
+constexpr int ROLE_TMP = NFIELDS;
+constexpr int ROLE_NONE = -1;
+__device__ __forceinline__ float* role_ptr(const RbScene& s, int role) {
+  return role == ROLE_TMP ? s.tmp : s.d.fld + (int64_t)role * s.d.fstride;
+}
+// The last scene j with start[j] <= g (start[0] = 0; scenes without work
+// units have start[j] == start[j+1] and are skipped).
+template <class I>
+__device__ __forceinline__ int find_scene(const I* __restrict__ start, int n, I g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+// lin_solve coefficients (a, 1/c) of a scene: velocity diffusion, density
+// diffusion, pressure (fluid.rs:286-294, 311, 356).
+__device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, float& cr) {
+  a = coef == 0 ? d.a_visc : coef == 1 ? d.a_diff : 1.0f;
+  cr = coef == 0 ? d.cr_visc : coef == 1 ? d.cr_diff : 0.25f;
+}
+
 // Red-black Gauss-Seidel with set_boundaries, T full sweeps per launch, all
 // in registers.  A workgroup of NW waves owns a region of 128 columns x
 // NW*R rows: lane l holds columns 2l and 2l+1 of its wave's band of R rows
@@ -81,33 +116,39 @@ enum BandSel : uint32_t {
 // buffer and the tile is written to the other (ping-pong).  Corners are set
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
-__global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict__ dp, const float* __restrict__ src,
-                                                     float* __restrict__ dst, const float* __restrict__ x0g,
-                                                     const uint8_t* __restrict__ code,
-                                                     const uint8_t* __restrict__ rowf, float a, float cr,
-                                                     int nsweeps) {
+__global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__ scs, const int* __restrict__ tstart,
+                                                     int nsc, int nt, int src_role, int dst_role, int x0_role,
+                                                     int coef, int nsweeps) {
   constexpr int H = 3 * T, W = 128, RH = NW * R;
   constexpr int TX = W - 2 * H, TY = RH - 2 * H;
   static_assert(TX > 0 && TY > 0 && TX % 2 == 0 && TY % 2 == 0 && R % 2 == 0, "band geometry");
   // [buffer][first / last row][column of the lane's pair][wave][lane]: one
   // float per lane and store, so a wave's 64 accesses hit 64 distinct banks
   __shared__ float edg[2][2][2][NW][64];
-  const SceneDesc& d = *dp;
-  const int S = d.S, N = d.N;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // Tile order: workgroups are dealt to the 8 XCDs round-robin by linear id;
-  // XCD k takes a contiguous run of row-major tiles instead, so tiles that
-  // share halo columns run together on one L2 (halves the HBM fetch).
-  int bx, by;
+  // XCD k takes a contiguous run of the row-major tiles (scene after scene)
+  // instead, so tiles that share halo columns run together on one L2.
+  int t;
   {
-    const int ntx = gridDim.x, nt = gridDim.x * gridDim.y;
-    const int L = blockIdx.x + blockIdx.y * ntx;
+    const int L = blockIdx.x;  // nt = tstart[nsc] workgroups
     const int per = nt >> 3, rem = nt & 7, xcd = L & 7;
-    const int t = xcd * per + min(xcd, rem) + (L >> 3);
-    bx = t % ntx;
-    by = t / ntx;
+    t = __builtin_amdgcn_readfirstlane(xcd * per + min(xcd, rem) + (L >> 3));
   }
+  const int js = __builtin_amdgcn_readfirstlane(find_scene(tstart, nsc, t));
+  const RbScene& sc = scs[js];
+  const SceneDesc& d = sc.d;
+  const int S = d.S, N = d.N;
+  const int tiles_x = (S + TX - 1) / TX;
+  const int bx = (t - tstart[js]) % tiles_x, by = (t - tstart[js]) / tiles_x;
+  const float* __restrict__ src = role_ptr(sc, src_role);
+  float* __restrict__ dst = role_ptr(sc, dst_role);
+  const float* __restrict__ x0g = role_ptr(sc, x0_role);
+  const uint8_t* __restrict__ code = d.code;
+  const uint8_t* __restrict__ rowf = d.rowf;
+  float a, cr;
+  coefs(d, coef, a, cr);
   const int gx0 = bx * TX - H;  // region origin; gx0 + gy0 is even (TX, TY, R even)
   const int gyb = by * TY - H;
   const int gy0 = gyb + w * R;          // this wave's band
@@ -262,28 +303,39 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const SceneDesc* __restrict
   }
 }
 
-// project, first half (fluid.rs:339-349)
-__global__ void k_div(const SceneDesc* __restrict__ dp, const float* __restrict__ u, const float* __restrict__ v,
-                      float* __restrict__ div, float* __restrict__ p) {
-  const SceneDesc& d = *dp;
-  const int S = d.S, N = d.N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+// project, first half (fluid.rs:339-349).  Grid: (column blocks of 256,
+// rows of all scenes); rstart = prefix of the scenes' interior rows.
+__global__ void k_div(const RbScene* __restrict__ scs, const int* __restrict__ rstart, int nsc, int u_role,
+                      int v_role, int div_role, int p_role) {
+  const int gr = blockIdx.y;
+  const int js = find_scene(rstart, nsc, gr);
+  const RbScene& sc = scs[js];
+  const int S = sc.d.S, N = sc.d.N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = gr - rstart[js] + 1;
   if (i > N) return;
+  const float* __restrict__ u = role_ptr(sc, u_role);
+  const float* __restrict__ v = role_ptr(sc, v_role);
   const int q = j * S + i;
-  div[q] = -0.5f * (u[q + 1] - u[q - 1] + v[q + S] - v[q - S]) / d.sf;
-  p[q] = 0.0f;
+  role_ptr(sc, div_role)[q] = -0.5f * (u[q + 1] - u[q - 1] + v[q + S] - v[q - S]) / sc.d.sf;
+  role_ptr(sc, p_role)[q] = 0.0f;
 }
 
 // project, second half (fluid.rs:364-371)
-__global__ void k_grad(const SceneDesc* __restrict__ dp, float* __restrict__ u, float* __restrict__ v,
-                       const float* __restrict__ p) {
-  const SceneDesc& d = *dp;
-  const int S = d.S, N = d.N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+__global__ void k_grad(const RbScene* __restrict__ scs, const int* __restrict__ rstart, int nsc, int u_role,
+                       int v_role, int p_role) {
+  const int gr = blockIdx.y;
+  const int js = find_scene(rstart, nsc, gr);
+  const RbScene& sc = scs[js];
+  const int S = sc.d.S, N = sc.d.N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = gr - rstart[js] + 1;
   if (i > N) return;
+  float* __restrict__ u = role_ptr(sc, u_role);
+  float* __restrict__ v = role_ptr(sc, v_role);
+  const float* __restrict__ p = role_ptr(sc, p_role);
+  const float sf = sc.d.sf;
   const int q = j * S + i;
-  u[q] = u[q] - 0.5f * (p[q + 1] - p[q - 1]) * d.sf;
-  v[q] = v[q] - 0.5f * (p[q + S] - p[q - S]) * d.sf;
+  u[q] = u[q] - 0.5f * (p[q + 1] - p[q - 1]) * sf;
+  v[q] = v[q] - 0.5f * (p[q + S] - p[q - S]) * sf;
 }
 
 struct Trace {
@@ -316,12 +368,18 @@ __device__ __forceinline__ float gather(const float* __restrict__ d0, const Trac
 
 // advect, pass A: first column of each row whose back-trace reaches the far
 // edge (the `break` of fluid.rs:420-423); N+1 when none.  One workgroup per
-// row; each thread tests 8 columns per round with all 16 loads in flight.
-__global__ void __launch_bounds__(256) k_adv_trig(const SceneDesc* __restrict__ dp, const float* __restrict__ u,
-                                                  const float* __restrict__ v, int* __restrict__ trig) {
-  const SceneDesc& d = *dp;
+// row of any scene; each thread tests 8 columns per round with all 16 loads
+// in flight.
+__global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ scs, const int* __restrict__ rstart,
+                                                  int nsc, int u_role, int v_role) {
+  const int gr = blockIdx.x;
+  const int js = find_scene(rstart, nsc, gr);
+  const RbScene& sc = scs[js];
+  const SceneDesc& d = sc.d;
   const int S = d.S, N = d.N;
-  const int j = blockIdx.x + 1;
+  const int j = gr - rstart[js] + 1;
+  const float* __restrict__ u = role_ptr(sc, u_role);
+  const float* __restrict__ v = role_ptr(sc, v_role);
   const int64_t row = (int64_t)j * S;
   constexpr int K = 8;
   int t = N + 1;
@@ -349,19 +407,23 @@ __global__ void __launch_bounds__(256) k_adv_trig(const SceneDesc* __restrict__ 
     if (threadIdx.x < w) red[threadIdx.x] = min(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) trig[j] = red[0];
+  if (threadIdx.x == 0) sc.trig[j] = red[0];
 }
 
-// advect, pass B (fluid.rs:398-430) for one or two fields sharing u, v.
-__global__ void k_adv(const SceneDesc* __restrict__ dp, float* __restrict__ o0, const float* __restrict__ i0,
-                      float* __restrict__ o1, const float* __restrict__ i1, const float* __restrict__ u,
-                      const float* __restrict__ v, const int* __restrict__ trig) {
-  const SceneDesc& d = *dp;
-  const int S = d.S, N = d.N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = blockIdx.y + 1;
+// advect, pass B (fluid.rs:398-430) for one or two fields sharing u, v
+// (o1_role ROLE_NONE: one field).
+__global__ void k_adv(const RbScene* __restrict__ scs, const int* __restrict__ rstart, int nsc, int o0_role,
+                      int i0_role, int o1_role, int i1_role, int u_role, int v_role) {
+  const int gr = blockIdx.y;
+  const int js = find_scene(rstart, nsc, gr);
+  const RbScene& sc = scs[js];
+  const int S = sc.d.S, N = sc.d.N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = gr - rstart[js] + 1;
   if (i > N) return;
-  const int tr = trig[j];
+  const int tr = sc.trig[j];
   if (i > tr) return;  // right of the break: buffer keeps its content
+  float* __restrict__ o0 = role_ptr(sc, o0_role);
+  float* __restrict__ o1 = o1_role == ROLE_NONE ? nullptr : role_ptr(sc, o1_role);
   const int q = j * S + i;
   if (i == tr && i == 1) {  // copies the ring cell d(0, j)
     o0[q] = o0[q - 1];
@@ -369,43 +431,57 @@ __global__ void k_adv(const SceneDesc* __restrict__ dp, float* __restrict__ o0, 
     return;
   }
   const int c = i == tr ? i - 1 : i;  // the trigger cell copies its left neighbour's value
-  const Trace r = back_trace(d, c, j, u[j * S + c], v[j * S + c]);
-  o0[q] = gather(i0, r, S);
-  if (o1) o1[q] = gather(i1, r, S);
+  const float* __restrict__ u = role_ptr(sc, u_role);
+  const float* __restrict__ v = role_ptr(sc, v_role);
+  const Trace r = back_trace(sc.d, c, j, u[j * S + c], v[j * S + c]);
+  o0[q] = gather(role_ptr(sc, i0_role), r, S);
+  if (o1) o1[q] = gather(role_ptr(sc, i1_role), r, S);
 }
 
-// set_boundaries as lists (built on the host from the mask), then corners.
-__global__ void k_bnd(const SceneDesc* __restrict__ dp, int mode0, float* __restrict__ x0, int mode1,
-                      float* __restrict__ x1) {
-  const SceneDesc& d = *dp;
-  const int off[3] = {0, d.nbnd[0], d.nbnd[0] + d.nbnd[1]};
-  const int n0 = d.nbnd[mode0], n1 = x1 ? d.nbnd[mode1] : 0;
+// set_boundaries as lists (built from the mask on the device): entry e of a
+// scene's mode0 list then of its mode1 list (role1 ROLE_NONE: none); bstart =
+// prefix of the scenes' entry counts for this (mode0, mode1).
+__global__ void k_bnd(const RbScene* __restrict__ scs, const int* __restrict__ bstart, int nsc, int mode0,
+                      int role0, int mode1, int role1) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n0) {
-    const int2 p = d.bnd[off[mode0] + e];
-    x0[p.x] = (mode0 == M_PASSIVE ? 1.0f : -1.0f) * x0[p.y];
-  } else if (e < n0 + n1) {
-    const int2 p = d.bnd[off[mode1] + e - n0];
-    x1[p.x] = (mode1 == M_PASSIVE ? 1.0f : -1.0f) * x1[p.y];
-  }
+  if (e >= bstart[nsc]) return;
+  const int js = find_scene(bstart, nsc, e);
+  const RbScene& sc = scs[js];
+  const SceneDesc& d = sc.d;
+  const int off[3] = {0, d.nbnd[0], d.nbnd[0] + d.nbnd[1]};
+  const int k = e - bstart[js];
+  const bool first = k < d.nbnd[mode0];
+  const int mode = first ? mode0 : mode1;
+  float* x = role_ptr(sc, first ? role0 : role1);
+  const int2 p = d.bnd[off[mode] + (first ? k : k - d.nbnd[mode0])];
+  x[p.x] = (mode == M_PASSIVE ? 1.0f : -1.0f) * x[p.y];
 }
-__global__ void k_corners(const SceneDesc* __restrict__ dp, float* __restrict__ x0, float* __restrict__ x1) {
-  const SceneDesc& d = *dp;
-  const int S = d.S;
-  float* x = threadIdx.x == 0 ? x0 : x1;
-  if (!x) return;
+
+// set_boundaries corners (fluid.rs:265-271): one thread per scene and field.
+__global__ void k_corners(const RbScene* __restrict__ scs, int nsc, int role0, int role1) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int js = t >> 1;
+  const int role = (t & 1) ? role1 : role0;
+  if (js >= nsc || role == ROLE_NONE) return;
+  const RbScene& sc = scs[js];
+  const int S = sc.d.S;
+  float* x = role_ptr(sc, role);
   x[0] = 0.5f * (x[1] + x[S]);
   x[(S - 1) * S] = 0.5f * (x[1 + (S - 1) * S] + x[(S - 2) * S]);
   x[S - 1] = 0.5f * (x[S - 2] + x[S - 1 + S]);
   x[S * S - 1] = 0.5f * (x[S - 2 + (S - 1) * S] + x[S - 1 + (S - 2) * S]);
 }
 
-__global__ void k_sources(const SceneDesc* __restrict__ dp) {
-  const SceneDesc& d = *dp;
-  const int64_t n = d.fstride;
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)d.S * d.S) return;
-  const int m = d.has_src;  // only the sources that were set (adding 0 would turn -0 into +0)
+// Frame sources (only the fields that were set; adding 0 would turn -0 into
+// +0) over every cell of every scene; cstart = prefix of the scenes' S*S.
+__global__ void k_sources(const RbScene* __restrict__ scs, const int64_t* __restrict__ cstart, int nsc) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= cstart[nsc]) return;
+  const int js = find_scene(cstart, nsc, g);
+  const SceneDesc& d = scs[js].d;
+  const int m = d.has_src;
+  if (!m) return;
+  const int64_t n = d.fstride, e = g - cstart[js];
   if (m & 1) {
     const float a = d.src[e];
     d.fld[F_D * n + e] += a;
@@ -415,21 +491,46 @@ __global__ void k_sources(const SceneDesc* __restrict__ dp) {
   if (m & 4) d.fld[F_V * n + e] += d.src[2 * n + e];
 }
 
-__global__ void k_copy(float* __restrict__ dst, const float* __restrict__ src, int64_t n) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < n) dst[e] = src[e];
+// dst = src over every cell of every scene.
+__global__ void k_copy(const RbScene* __restrict__ scs, const int64_t* __restrict__ cstart, int nsc, int dst_role,
+                       int src_role) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= cstart[nsc]) return;
+  const int js = find_scene(cstart, nsc, g);
+  const RbScene& sc = scs[js];
+  const int64_t e = g - cstart[js];
+  role_ptr(sc, dst_role)[e] = role_ptr(sc, src_role)[e];
 }
 
 }  // namespace rb
 
 // ------------------------------------------------------------------ host --
-struct RBWork {
-  float* tmp = nullptr;   // ping-pong field
-  int* trig = nullptr;    // per-row break column
+using rb::RbScene;
+// A launch group: its scenes and the prefix tables of their work units, in
+// one device buffer written before any frame is captured.
+struct RbGroup {
+  std::vector<eqf_scene*> scenes;  // largest first
+  hipStream_t stream = nullptr;    // null: each frame runs on scenes[0]'s stream
+  void* d_tab = nullptr;
+  size_t tab_bytes = 0;
+  RbScene* d_scs = nullptr;
+  int *d_tstart = nullptr, *d_rstart = nullptr, *d_bpp = nullptr, *d_brc = nullptr, *d_bp = nullptr;
+  int64_t* d_cstart = nullptr;
+  int n = 0, rows = 0, maxN = 0, tiles = 0, bpp = 0, brc = 0, bp = 0;
+  int64_t cells = 0;
+  bool any_src = false;
+  int diff_iters = 0, press_iters = 0;
+  int tile = 0, tsweeps = 4;  // band shape index (kBands) and sweeps per launch
+  int tab_tile = -1, tab_T = -1;
   hipGraphExec_t graph = nullptr;
-  int tile = 0;           // band shape index (kBands)
-  int tsweeps = 4;        // sweeps per launch
-  uint64_t key = 0;       // parameters the graph was built for
+  int64_t graph_frames = 0;
+  uint64_t key = 0;  // what the table and the graph were built for
+};
+
+struct RBWork {
+  float* tmp = nullptr;  // ping-pong field
+  int* trig = nullptr;   // per-row break columns
+  RbGroup one;           // the scene alone (eqf_step)
 };
 
 // Band shapes (waves per workgroup x rows per wave); the region is always 128
@@ -439,106 +540,251 @@ struct BandCfg {
 };
 static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}};
 
+static int band_tiles(int S, int tile, int T) {
+  const BandCfg b = kBands[tile];
+  const int TX = 128 - 6 * T, TY = b.nw * b.r - 6 * T;
+  return ((S + TX - 1) / TX) * ((S + TY - 1) / TY);
+}
+
+static uint64_t group_key(const RbGroup& g) {
+  uint64_t key = 1469598103934665603ull;
+  auto mix = [&key](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+  for (const eqf_scene* s : g.scenes) {
+    const SceneDesc& d = s->desc;
+    mix((uint64_t)(uintptr_t)s);
+    mix((uint64_t)d.diff_iters);
+    mix((uint64_t)d.press_iters);
+    mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_visc));
+    mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_diff));
+    mix((uint64_t)d.has_src);
+    mix((uint64_t)d.nbnd[0] | ((uint64_t)d.nbnd[1] << 21) | ((uint64_t)d.nbnd[2] << 42));
+  }
+  mix((uint64_t)g.tile * 16 + (uint64_t)g.tsweeps);
+  return key;
+}
+
+// (Re)writes the group's tables from its scenes' descriptors.
+static int group_tables(RbGroup& g, std::string& err) {
+  const int n = (int)g.scenes.size();
+  std::vector<RbScene> scs(n);
+  std::vector<int> tstart(n + 1, 0), rstart(n + 1, 0), bpp(n + 1, 0), brc(n + 1, 0), bp(n + 1, 0);
+  std::vector<int64_t> cstart(n + 1, 0);
+  g.any_src = false;
+  g.maxN = 0;
+  for (int i = 0; i < n; ++i) {
+    eqf_scene* s = g.scenes[i];
+    auto* w = static_cast<RBWork*>(s->rb);
+    scs[i] = RbScene{s->desc, w->tmp, w->trig};
+    tstart[i + 1] = tstart[i] + band_tiles(s->S, g.tile, g.tsweeps);
+    rstart[i + 1] = rstart[i] + s->N;
+    cstart[i + 1] = cstart[i] + (int64_t)s->S * s->S;
+    bpp[i + 1] = bpp[i] + 2 * s->nbnd[M_PASSIVE];
+    brc[i + 1] = brc[i] + s->nbnd[M_ROW] + s->nbnd[M_COL];
+    bp[i + 1] = bp[i] + s->nbnd[M_PASSIVE];
+    g.any_src = g.any_src || s->desc.has_src;
+    g.maxN = std::max(g.maxN, s->N);
+  }
+  g.n = n;
+  g.tiles = tstart[n];
+  g.rows = rstart[n];
+  g.cells = cstart[n];
+  g.bpp = bpp[n];
+  g.brc = brc[n];
+  g.bp = bp[n];
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t b_scs = up(sizeof(RbScene) * n), b_i = up(sizeof(int) * (n + 1)), b_c = up(sizeof(int64_t) * (n + 1));
+  const size_t bytes = b_scs + 5 * b_i + b_c;
+  if (bytes > g.tab_bytes) {
+    (void)hipFree(g.d_tab);
+    g.d_tab = nullptr;
+    g.tab_bytes = 0;
+    hipError_t e = hipMalloc(&g.d_tab, bytes);
+    if (e != hipSuccess) {
+      err = std::string("red-black tables: ") + hipGetErrorString(e);
+      return EQF_ENOMEM;
+    }
+    g.tab_bytes = bytes;
+  }
+  std::vector<char> h(bytes, 0);
+  char* base = h.data();
+  std::memcpy(base, scs.data(), sizeof(RbScene) * n);
+  int* ib[5] = {tstart.data(), rstart.data(), bpp.data(), brc.data(), bp.data()};
+  for (int k = 0; k < 5; ++k) std::memcpy(base + b_scs + k * b_i, ib[k], sizeof(int) * (n + 1));
+  std::memcpy(base + b_scs + 5 * b_i, cstart.data(), sizeof(int64_t) * (n + 1));
+  char* d = (char*)g.d_tab;
+  g.d_scs = (RbScene*)d;
+  g.d_tstart = (int*)(d + b_scs);
+  g.d_rstart = (int*)(d + b_scs + b_i);
+  g.d_bpp = (int*)(d + b_scs + 2 * b_i);
+  g.d_brc = (int*)(d + b_scs + 3 * b_i);
+  g.d_bp = (int*)(d + b_scs + 4 * b_i);
+  g.d_cstart = (int64_t*)(d + b_scs + 5 * b_i);
+  hipError_t e = hipMemcpy(g.d_tab, h.data(), bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    err = std::string("red-black tables: ") + hipGetErrorString(e);
+    return EQF_EDEVICE;
+  }
+  g.tab_tile = g.tile;
+  g.tab_T = g.tsweeps;
+  return EQF_OK;
+}
+
 template <int MODE, int T, int NW, int R>
-static void launch_band(hipStream_t st, const SceneDesc* dd, const float* src, float* dst, const float* x0,
-                        const uint8_t* code, const uint8_t* rowf, float a, float cr, int nsw, int S) {
-  constexpr int TX = 128 - 6 * T, TY = NW * R - 6 * T;
-  dim3 grid((S + TX - 1) / TX, (S + TY - 1) / TY);
-  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), grid, dim3(64 * NW), 0, st, dd, src, dst, x0, code, rowf, a, cr,
-                     nsw);
+static void launch_band(hipStream_t st, const RbGroup& g, int src, int dst, int x0, int coef, int nsw) {
+  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), dim3(g.tiles), dim3(64 * NW), 0, st, g.d_scs, g.d_tstart, g.n,
+                     g.tiles, src, dst, x0, coef, nsw);
 }
 
 template <int MODE, int NW, int R>
-static void launch_band_t(hipStream_t st, int T, const SceneDesc* dd, const float* src, float* dst,
-                          const float* x0, const uint8_t* code, const uint8_t* rowf, float a, float cr, int nsw,
-                          int S) {
-  if (T == 1) launch_band<MODE, 1, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else if (T == 2) launch_band<MODE, 2, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else if (T == 4) launch_band<MODE, 4, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else if (T == 5) launch_band<MODE, 5, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else if (T == 6) launch_band<MODE, 6, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else launch_band<MODE, 8, NW, R>(st, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+static void launch_band_t(hipStream_t st, const RbGroup& g, int src, int dst, int x0, int coef, int nsw) {
+  const int T = g.tsweeps;
+  if (T == 1) launch_band<MODE, 1, NW, R>(st, g, src, dst, x0, coef, nsw);
+  else if (T == 2) launch_band<MODE, 2, NW, R>(st, g, src, dst, x0, coef, nsw);
+  else if (T == 4) launch_band<MODE, 4, NW, R>(st, g, src, dst, x0, coef, nsw);
+  else if (T == 5) launch_band<MODE, 5, NW, R>(st, g, src, dst, x0, coef, nsw);
+  else if (T == 6) launch_band<MODE, 6, NW, R>(st, g, src, dst, x0, coef, nsw);
+  else launch_band<MODE, 8, NW, R>(st, g, src, dst, x0, coef, nsw);
 }
 
 template <int MODE>
-static void launch_gs_mode(hipStream_t st, int tile, int T, const SceneDesc* dd, const float* src, float* dst,
-                           const float* x0, const uint8_t* code, const uint8_t* rowf, float a, float cr, int nsw,
-                           int S) {
-  if (tile == 1) launch_band_t<MODE, 8, 32>(st, T, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else if (tile == 2) launch_band_t<MODE, 16, 16>(st, T, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
-  else launch_band_t<MODE, 8, 16>(st, T, dd, src, dst, x0, code, rowf, a, cr, nsw, S);
+static void launch_gs_mode(hipStream_t st, const RbGroup& g, int src, int dst, int x0, int coef, int nsw) {
+  if (g.tile == 1) launch_band_t<MODE, 8, 32>(st, g, src, dst, x0, coef, nsw);
+  else if (g.tile == 2) launch_band_t<MODE, 16, 16>(st, g, src, dst, x0, coef, nsw);
+  else launch_band_t<MODE, 8, 16>(st, g, src, dst, x0, coef, nsw);
 }
 
-// lin_solve in red-black order: ceil(iters/T) tile passes, ping-ponging
-// between x and the scratch field; an odd pass count ends with a copy back.
-static void rb_lin_solve(eqf_scene* s, RBWork* w, int mode, float* x, const float* x0, float a, float cr,
-                         int iters) {
+static void g_copy(hipStream_t st, const RbGroup& g, int dst, int src) {
+  hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((g.cells + 255) / 256)), dim3(256), 0, st, g.d_scs, g.d_cstart, g.n,
+                     dst, src);
+}
+
+// lin_solve in red-black order on every scene of the group (one iteration
+// count): ceil(iters/T) tile passes ping-ponging between x and the scratch
+// field; an odd pass count ends with a copy back; then the corners of the
+// last set_boundaries (fluid.rs:265-271).
+static void g_lin_solve(hipStream_t st, const RbGroup& g, int mode, int x, int x0, int coef, int iters) {
   if (iters <= 0) return;
-  const int S = s->S, T = w->tsweeps;
-  const SceneDesc* dd = s->d_desc;
-  float* a_ = x;
-  float* b_ = w->tmp;
+  int a_ = x, b_ = rb::ROLE_TMP;
   for (int done = 0; done < iters;) {
-    const int nsw = std::min(T, iters - done);
-    const uint8_t *cd = s->d_code, *rf = s->d_rowf;
-    if (mode == M_ROW) launch_gs_mode<M_ROW>(s->stream, w->tile, T, dd, a_, b_, x0, cd, rf, a, cr, nsw, S);
-    else if (mode == M_COL) launch_gs_mode<M_COL>(s->stream, w->tile, T, dd, a_, b_, x0, cd, rf, a, cr, nsw, S);
-    else launch_gs_mode<M_PASSIVE>(s->stream, w->tile, T, dd, a_, b_, x0, cd, rf, a, cr, nsw, S);
+    const int nsw = std::min(g.tsweeps, iters - done);
+    if (mode == M_ROW) launch_gs_mode<M_ROW>(st, g, a_, b_, x0, coef, nsw);
+    else if (mode == M_COL) launch_gs_mode<M_COL>(st, g, a_, b_, x0, coef, nsw);
+    else launch_gs_mode<M_PASSIVE>(st, g, a_, b_, x0, coef, nsw);
     std::swap(a_, b_);
     done += nsw;
   }
-  if (a_ != x) {
-    const int64_t n = (int64_t)S * S;
-    hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->stream, x, a_, n);
-  }
-  // corners of the last set_boundaries (fluid.rs:265-271)
-  hipLaunchKernelGGL(rb::k_corners, dim3(1), dim3(2), 0, s->stream, s->d_desc, x, (float*)nullptr);
+  if (a_ != x) g_copy(st, g, x, a_);
+  hipLaunchKernelGGL(rb::k_corners, dim3((unsigned)((2 * g.n + 255) / 256)), dim3(256), 0, st, g.d_scs, g.n, x,
+                     rb::ROLE_NONE);
 }
 
-static void rb_bnd(eqf_scene* s, int mode0, float* x0, int mode1, float* x1) {
-  const int n = s->nbnd[mode0] + (x1 ? s->nbnd[mode1] : 0);
-  if (n > 0)
-    hipLaunchKernelGGL(rb::k_bnd, dim3((n + 255) / 256), dim3(256), 0, s->stream, s->d_desc, mode0, x0, mode1, x1);
-  hipLaunchKernelGGL(rb::k_corners, dim3(1), dim3(2), 0, s->stream, s->d_desc, x0, x1);
+static void g_bnd(hipStream_t st, const RbGroup& g, int mode0, int x0, int mode1, int x1) {
+  const int* tab = x1 == rb::ROLE_NONE ? g.d_bp : mode0 == M_PASSIVE ? g.d_bpp : g.d_brc;
+  const int total = x1 == rb::ROLE_NONE ? g.bp : mode0 == M_PASSIVE ? g.bpp : g.brc;
+  if (total > 0)
+    hipLaunchKernelGGL(rb::k_bnd, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g.d_scs, tab, g.n, mode0,
+                       x0, mode1, x1);
+  hipLaunchKernelGGL(rb::k_corners, dim3((unsigned)((2 * g.n + 255) / 256)), dim3(256), 0, st, g.d_scs, g.n, x0, x1);
 }
 
-static void rb_frame(eqf_scene* s, RBWork* w) {
-  const int S = s->S, N = s->N;
-  const SceneDesc& d = s->desc;
-  float* f = s->d_fld;
-  const int64_t n = s->fstride;
-  float *Dn = f + F_D * n, *Sc = f + F_SC * n, *U = f + F_U * n, *V = f + F_V * n, *U0 = f + F_U0 * n,
-        *V0 = f + F_V0 * n;
-  hipStream_t st = s->stream;
-  const dim3 rows((N + 255) / 256, N);
-  if (d.has_src)
-    hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)(((int64_t)S * S + 255) / 256)), dim3(256), 0, st, s->d_desc);
-  rb_lin_solve(s, w, M_ROW, U0, U, d.a_visc, d.cr_visc, d.diff_iters);  // fluid.rs:438
-  rb_lin_solve(s, w, M_COL, V0, V, d.a_visc, d.cr_visc, d.diff_iters);  // fluid.rs:448
+// One frame (fluid.rs:437-524) of every scene of the group.  All scenes share
+// the iteration counts, so the sequence of launches is the same for all.
+static void g_frame(hipStream_t st, const RbGroup& g) {
+  using rb::ROLE_NONE;
+  const dim3 rows((unsigned)((g.maxN + 255) / 256), (unsigned)g.rows);
+  if (g.any_src)
+    hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)((g.cells + 255) / 256)), dim3(256), 0, st, g.d_scs, g.d_cstart,
+                       g.n);
+  g_lin_solve(st, g, M_ROW, F_U0, F_U, 0, g.diff_iters);  // fluid.rs:438
+  g_lin_solve(st, g, M_COL, F_V0, F_V, 0, g.diff_iters);  // fluid.rs:448
   // project(U0, V0, p=U, div=V), fluid.rs:459
-  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, s->d_desc, U0, V0, V, U);
-  rb_bnd(s, M_PASSIVE, V, M_PASSIVE, U);
-  rb_lin_solve(s, w, M_PASSIVE, U, V, 1.0f, 0.25f, d.press_iters);
-  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, s->d_desc, U0, V0, U);
-  rb_bnd(s, M_ROW, U0, M_COL, V0);
+  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0, F_V, F_U);
+  g_bnd(st, g, M_PASSIVE, F_V, M_PASSIVE, F_U);
+  g_lin_solve(st, g, M_PASSIVE, F_U, F_V, 2, g.press_iters);
+  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0, F_U);
+  g_bnd(st, g, M_ROW, F_U0, M_COL, F_V0);
   // advect U from U0 and V from V0 with (U0, V0), fluid.rs:469-489
-  hipLaunchKernelGGL(rb::k_adv_trig, dim3(N), dim3(256), 0, st, s->d_desc, U0, V0, w->trig);
-  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, s->d_desc, U, U0, V, V0, U0, V0, w->trig);
-  rb_bnd(s, M_ROW, U, M_COL, V);
+  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0);
+  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_U0, F_V, F_V0, F_U0, F_V0);
+  g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // project(U, V, p=U0, div=V0), fluid.rs:491
-  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, s->d_desc, U, V, V0, U0);
-  rb_bnd(s, M_PASSIVE, V0, M_PASSIVE, U0);
-  rb_lin_solve(s, w, M_PASSIVE, U0, V0, 1.0f, 0.25f, d.press_iters);
-  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, s->d_desc, U, V, U0);
-  rb_bnd(s, M_ROW, U, M_COL, V);
+  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V, F_V0, F_U0);
+  g_bnd(st, g, M_PASSIVE, F_V0, M_PASSIVE, F_U0);
+  g_lin_solve(st, g, M_PASSIVE, F_U0, F_V0, 2, g.press_iters);
+  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V, F_U0);
+  g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // density, fluid.rs:501-523
-  rb_lin_solve(s, w, M_PASSIVE, Sc, Dn, d.a_diff, d.cr_diff, d.diff_iters);
-  hipLaunchKernelGGL(rb::k_adv_trig, dim3(N), dim3(256), 0, st, s->d_desc, U, V, w->trig);
-  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, s->d_desc, Dn, Sc, (float*)nullptr, (const float*)nullptr,
-                     U, V, w->trig);
-  rb_bnd(s, M_PASSIVE, Dn, M_PASSIVE, nullptr);
-  const int64_t nn = (int64_t)S * S;
-  hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, Sc, Dn, nn);
+  g_lin_solve(st, g, M_PASSIVE, F_SC, F_D, 1, g.diff_iters);
+  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V);
+  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_D, F_SC, ROLE_NONE, ROLE_NONE,
+                     F_U, F_V);
+  g_bnd(st, g, M_PASSIVE, F_D, M_PASSIVE, ROLE_NONE);
+  g_copy(st, g, F_SC, F_D);
+}
+
+static void group_release(RbGroup& g) {
+  if (g.graph) (void)hipGraphExecDestroy(g.graph);
+  g.graph = nullptr;
+  (void)hipFree(g.d_tab);
+  g.d_tab = nullptr;
+  g.tab_bytes = 0;
+}
+
+// Advances every scene of the group by nframes on `st`: the frames are
+// captured once as a graph (rebuilt when the scenes' parameters, lists or
+// the tiling change) and replayed.
+static int group_step(RbGroup& g, hipStream_t st, int64_t nframes, std::string& err) {
+  if (nframes <= 0) return EQF_OK;
+  g.diff_iters = g.scenes[0]->desc.diff_iters;
+  g.press_iters = g.scenes[0]->desc.press_iters;
+  for (const eqf_scene* s : g.scenes)
+    if (s->desc.diff_iters != g.diff_iters || s->desc.press_iters != g.press_iters) {
+      err = "red-black scenes of one batch need the same iteration counts";
+      return EQF_EINVAL;
+    }
+  const uint64_t key = group_key(g);
+  if (!g.d_tab || key != g.key || g.tab_tile != g.tile || g.tab_T != g.tsweeps) {
+    int rc = group_tables(g, err);
+    if (rc) return rc;
+    if (g.graph) (void)hipGraphExecDestroy(g.graph);
+    g.graph = nullptr;
+    g.key = key;
+  }
+  // one frame per graph, replayed nframes times (a frame is ~100 launches)
+  const int64_t per = 1;
+  if (!g.graph || g.graph_frames != per) {
+    if (g.graph) (void)hipGraphExecDestroy(g.graph);
+    g.graph = nullptr;
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      err = std::string("capture: ") + hipGetErrorString(e);
+      return EQF_EDEVICE;
+    }
+    for (int64_t f = 0; f < per; ++f) g_frame(st, g);
+    hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(st, &gr);
+    if (le != hipSuccess || e != hipSuccess) {
+      err = std::string("red-black frame capture: ") + hipGetErrorString(le != hipSuccess ? le : e);
+      if (gr) (void)hipGraphDestroy(gr);
+      return EQF_EDEVICE;
+    }
+    e = hipGraphInstantiate(&g.graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) {
+      err = std::string("graph instantiate: ") + hipGetErrorString(e);
+      return EQF_EDEVICE;
+    }
+    g.graph_frames = per;
+  }
+  for (int64_t f = 0; f < nframes; f += per) {
+    hipError_t e = hipGraphLaunch(g.graph, st);
+    if (e != hipSuccess) {
+      err = std::string("graph launch: ") + hipGetErrorString(e);
+      return EQF_EDEVICE;
+    }
+  }
+  return EQF_OK;
 }
 
 void* rb_create(eqf_scene* s, std::string& err) {
@@ -551,84 +797,66 @@ void* rb_create(eqf_scene* s, std::string& err) {
     rb_destroy(w);
     return nullptr;
   }
-  // Default tiling; the autotuner (eqf_scene_tune) may override it.
-  w->tile = 0;
-  w->tsweeps = 4;
-  if (const char* t = getenv("EQF_RB_TSWEEPS")) {
-    const int v = atoi(t);
-    w->tsweeps = v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1;
-  }
-  if (const char* t = getenv("EQF_RB_TILE")) w->tile = std::max(0, std::min(2, atoi(t)));
+  w->one.scenes = {s};
+  w->one.tile = 0;  // the autotuner (equilibrium_amd/tune.py) may override these
+  w->one.tsweeps = 4;
   return w;
 }
 
 void rb_destroy(void* p) {
   auto* w = static_cast<RBWork*>(p);
   if (!w) return;
-  if (w->graph) (void)hipGraphExecDestroy(w->graph);
+  group_release(w->one);
   (void)hipFree(w->tmp);
   (void)hipFree(w->trig);
   delete w;
 }
 
+static bool tiling_ok(int tile, int tsweeps) {
+  return tile >= 0 && tile <= 2 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
+}
+
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {
   auto* w = static_cast<RBWork*>(s->rb);
-  if (!w) return EQF_EINVAL;
-  if (tile < 0 || tile > 2 || tsweeps < 1 || tsweeps > 8 || tsweeps == 3 || tsweeps == 7) return EQF_EINVAL;
-  if (w->tile != tile || w->tsweeps != tsweeps) {
-    w->tile = tile;
-    w->tsweeps = tsweeps;
-    w->key = 0;
-  }
+  if (!w || !tiling_ok(tile, tsweeps)) return EQF_EINVAL;
+  w->one.tile = tile;
+  w->one.tsweeps = tsweeps;
   return EQF_OK;
 }
 
 int rb_step(eqf_scene* s, int64_t nframes, std::string& err) {
   auto* w = static_cast<RBWork*>(s->rb);
-  // Parameters are baked into the graph: rebuild when they change.
-  const SceneDesc& d = s->desc;
-  uint64_t key = 1469598103934665603ull;
-  auto mix = [&key](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
-  mix((uint64_t)d.diff_iters);
-  mix((uint64_t)d.press_iters);
-  mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_visc));
-  mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_diff));
-  mix((uint64_t)d.has_src);
-  mix((uint64_t)d.nbnd[0] | ((uint64_t)d.nbnd[1] << 21) | ((uint64_t)d.nbnd[2] << 42));
-  mix((uint64_t)w->tile * 16 + (uint64_t)w->tsweeps);
-  if (!w->graph || w->key != key) {
-    if (w->graph) (void)hipGraphExecDestroy(w->graph);
-    w->graph = nullptr;
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) {
-      err = std::string("capture: ") + hipGetErrorString(e);
-      return EQF_EDEVICE;
+  return group_step(w->one, s->stream, nframes, err);
+}
+
+// Batches of red-black scenes (eqf_runtime.hip): one group over all of them.
+void* rb_batch_create(const std::vector<eqf_scene*>& scenes, hipStream_t st, std::string& err) {
+  for (const eqf_scene* s : scenes)
+    if (s->desc.diff_iters != scenes[0]->desc.diff_iters || s->desc.press_iters != scenes[0]->desc.press_iters) {
+      err = "red-black scenes of one batch need the same iteration counts";
+      return nullptr;
     }
-    rb_frame(s, w);
-    hipError_t le = hipGetLastError();
-    e = hipStreamEndCapture(s->stream, &g);
-    if (le != hipSuccess || e != hipSuccess) {
-      err = std::string("red-black frame capture: ") + hipGetErrorString(le != hipSuccess ? le : e);
-      if (g) (void)hipGraphDestroy(g);
-      return EQF_EDEVICE;
-    }
-    e = hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-      err = std::string("graph instantiate: ") + hipGetErrorString(e);
-      return EQF_EDEVICE;
-    }
-    w->key = key;
-  }
-  for (int64_t f = 0; f < nframes; ++f) {
-    hipError_t e = hipGraphLaunch(w->graph, s->stream);
-    if (e != hipSuccess) {
-      err = std::string("graph launch: ") + hipGetErrorString(e);
-      return EQF_EDEVICE;
-    }
-  }
+  auto* g = new RbGroup;
+  g->scenes = scenes;
+  g->stream = st;
+  return g;
+}
+void rb_batch_destroy(void* p) {
+  auto* g = static_cast<RbGroup*>(p);
+  if (!g) return;
+  group_release(*g);
+  delete g;
+}
+int rb_batch_set_tiling(void* p, int tile, int tsweeps) {
+  auto* g = static_cast<RbGroup*>(p);
+  if (!g || !tiling_ok(tile, tsweeps)) return EQF_EINVAL;
+  g->tile = tile;
+  g->tsweeps = tsweeps;
   return EQF_OK;
+}
+int rb_batch_step(void* p, int64_t nframes, std::string& err) {
+  auto* g = static_cast<RbGroup*>(p);
+  return group_step(*g, g->stream, nframes, err);
 }
 
 }  // namespace eqf

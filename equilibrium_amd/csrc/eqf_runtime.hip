@@ -538,6 +538,7 @@ struct eqf_batch {
   int W = 1;
   int max_waves = 16;  // tuning: waves per workgroup at most (>= the largest strip count)
   int order = EQF_BATCH_LARGEST_FIRST;
+  void* rbg = nullptr;  // red-black batch: the launch group over all scenes (eqf_redblack.hip)
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   int64_t graph_frames = 0;
@@ -580,7 +581,8 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
   if (!ctx || !scenes || n <= 0 || !out) return fail(EQF_EINVAL, "bad arguments");
   for (int i = 0; i < n; ++i) {
     if (!scenes[i]) return fail(EQF_EINVAL, "NULL scene");
-    if (scenes[i]->layout != L_CYCLIC) return fail(EQF_EINVAL, "batches hold wavefront-mode scenes");
+    if (scenes[i]->layout != scenes[0]->layout)
+      return fail(EQF_EINVAL, "a batch holds scenes of one mode (wavefront or red-black)");
     if (scenes[i]->in_batch) return fail(EQF_EBUSY, "scene already in a batch");
   }
   HIPCHK(hipSetDevice(ctx->device));
@@ -596,9 +598,24 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
   if (e == hipSuccess) e = hipMalloc(&b->d_descs, sizeof(SceneDesc) * n);
   if (e == hipSuccess) e = hipEventCreate(&b->ev0);
   if (e == hipSuccess) e = hipEventCreate(&b->ev1);
-  int rc = e == hipSuccess ? batch_layout(b) : fail(EQF_EDEVICE, std::string("batch create: ") + hipGetErrorString(e));
+  int rc = EQF_OK;
+  if (e != hipSuccess) {
+    rc = fail(EQF_EDEVICE, std::string("batch create: ") + hipGetErrorString(e));
+  } else if (scenes[0]->layout == L_ROWMAJOR) {
+    // red-black: every frame kernel covers all scenes, largest first
+    b->scenes = b->given;
+    std::stable_sort(b->scenes.begin(), b->scenes.end(),
+                     [](const eqf_scene* a, const eqf_scene* c) { return a->N > c->N; });
+    std::string err;
+    b->rbg = rb_batch_create(b->scenes, b->stream, err);
+    if (!b->rbg) rc = fail(EQF_EINVAL, err);
+  } else {
+    rc = batch_layout(b);
+  }
   if (rc) {
     (void)hipFree(b->d_descs);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return rc;
@@ -608,8 +625,19 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
   return EQF_OK;
 }
 
+extern "C" int eqf_batch_set_band(eqf_batch* b, int tile, int sweeps_per_block) {
+  if (!b) return fail(EQF_EINVAL, "batch is NULL");
+  if (!b->rbg) return fail(EQF_EINVAL, "band tiling applies to red-black batches");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (rb_batch_set_tiling(b->rbg, tile, sweeps_per_block) != EQF_OK)
+    return fail(EQF_EINVAL, "tile must be 0..2 and sweeps 1, 2, 4, 5, 6 or 8");
+  return EQF_OK;
+}
+
 extern "C" int eqf_batch_set_tuning(eqf_batch* b, int max_waves, int order) {
   if (!b) return fail(EQF_EINVAL, "batch is NULL");
+  if (b->rbg) return fail(EQF_EINVAL, "red-black batches are tuned with eqf_batch_set_band");
   if (max_waves < 1 || max_waves > 16) return fail(EQF_EINVAL, "max_waves must be 1..16");
   if (order != EQF_BATCH_LARGEST_FIRST && order != EQF_BATCH_GIVEN_ORDER) return fail(EQF_EINVAL, "bad order");
   HIPCHK(hipSetDevice(b->ctx->device));
@@ -662,6 +690,14 @@ extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {
   if (nframes < 0) return fail(EQF_EINVAL, "nframes < 0");
   if (nframes == 0) return EQF_OK;
   HIPCHK(hipSetDevice(b->ctx->device));
+  if (b->rbg) {  // red-black: the group's frames, captured once per nframes
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    std::string err;
+    const int rc = rb_batch_step(b->rbg, nframes, err);
+    if (rc) return fail(rc, err);
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    return EQF_OK;
+  }
   // One launch per frame, captured once as a graph of `nframes` launches.
   if (!b->gexec || b->graph_frames != nframes) {
     if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
@@ -701,6 +737,7 @@ extern "C" int eqf_batch_destroy(eqf_batch* b) {
   if (!b) return EQF_OK;
   (void)hipStreamSynchronize(b->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  rb_batch_destroy(b->rbg);
   for (auto* s : b->scenes) s->in_batch = 0;
   (void)hipFree(b->d_descs);
   (void)hipEventDestroy(b->ev0);

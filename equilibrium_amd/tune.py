@@ -4,6 +4,8 @@
 * Batches (wavefront mode): waves per workgroup at most -- with fewer waves a
   strip runs fewer sweep streams -- and the scene-to-workgroup order
   (eqf_batch_set_tuning).
+* Batches (red-black mode): band shape x sweeps per launch over all scenes
+  (eqf_batch_set_band).
 
 Each times frames per configuration on the GPU (on clones, so the tuned
 scenes are not advanced) and keeps the fastest; results are cached in a JSON
@@ -107,7 +109,7 @@ def tune_batch(scenes, frames=2, use_cache=True):
                 continue
             for o in BATCH_ORDERS:
                 b.set_tuning(w, o)
-                b.step(1)  # builds the graph
+                b.step(frames)  # builds the graph of `frames` launches
                 b.sync()
                 t0 = time.perf_counter()
                 b.step(frames)
@@ -124,3 +126,38 @@ def tune_batch(scenes, frames=2, use_cache=True):
     if use_cache:
         _save(db)
     return w, o, timings
+
+
+def tune_batch_band(scenes, frames=2, use_cache=True):
+    """Picks (tile, sweeps_per_block) for a batch of red-black _abi.Scene
+    objects; returns (tile, sweeps, timings in ms per frame).  Trials run on
+    clones."""
+    key = "rb" + _batch_key(scenes)
+    db = _load() if use_cache else {}
+    if key in db:
+        tile, tsw = db[key]["best"]
+        return tile, tsw, db[key].get("ms", {})
+    clones = [s.clone() for s in scenes]
+    b = _abi.Batch(clones)
+    timings = {}
+    try:
+        for tile in TILES:
+            for tsw in SWEEPS_PER_BLOCK:
+                b.set_band(tile, tsw)
+                b.step(frames)  # builds the tables and the graph
+                b.sync()
+                t0 = time.perf_counter()
+                b.step(frames)
+                b.sync()
+                timings[f"{tile}:{tsw}"] = (time.perf_counter() - t0) * 1e3 / frames
+    finally:
+        b.close()
+        for c in clones:
+            c.close()
+    best = min(timings, key=timings.get)
+    tile, tsw = (int(x) for x in best.split(":"))
+    db = _load() if use_cache else {}
+    db[key] = {"best": [tile, tsw], "ms": timings, "tiles": {str(k): v for k, v in TILES.items()}}
+    if use_cache:
+        _save(db)
+    return tile, tsw, timings

@@ -138,10 +138,13 @@ int eqf_sync(eqf_scene *s);
  * Chosen by the autotuner (equilibrium_amd/tune.py). */
 int eqf_scene_set_tuning(eqf_scene *s, int tile, int sweeps_per_block);
 
-/* Batches: many scenes (any sizes, wavefront mode) advanced by one fused
- * launch per frame; K frames are captured once as a HIP graph and replayed.
- * Scenes stay owned by the caller but must not be stepped separately while
- * in a batch. */
+/* Batches: many scenes of one mode advanced together; K frames are captured
+ * once as a HIP graph and replayed.  Wavefront mode (sizes up to 1026): one
+ * fused launch per frame runs every scene's whole frame.  Red-black mode (any
+ * sizes, one iteration count): every kernel of the frame covers all scenes
+ * (band passes over all scenes' tiles, then the grid-wide phases).  Scenes
+ * stay owned by the caller but must not be stepped separately while in a
+ * batch. */
 int eqf_batch_create(eqf_ctx *ctx, eqf_scene *const *scenes, int n, eqf_batch **out);
 int eqf_batch_step(eqf_batch *b, int64_t nframes);
 int eqf_batch_sync(eqf_batch *b);
@@ -155,6 +158,9 @@ int eqf_batch_last_ms(const eqf_batch *b, double *ms);
  * autotuner (equilibrium_amd/tune.py). */
 enum { EQF_BATCH_LARGEST_FIRST = 0, EQF_BATCH_GIVEN_ORDER = 1 };
 int eqf_batch_set_tuning(eqf_batch *b, int max_waves, int order);
+/* Band shape and sweeps per launch of a red-black batch (as
+ * eqf_scene_set_tuning); chosen by the autotuner. */
+int eqf_batch_set_band(eqf_batch *b, int tile, int sweeps_per_block);
 
 /* Serving: continuous batching of scene requests (no counterpart in the
  * reference, whose frame loop runs one scene; renderer_helpers.rs:52-72).

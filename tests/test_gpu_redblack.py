@@ -85,3 +85,58 @@ def test_redblack_accuracy_vs_fp64():
     print("[rb-fp64] worst per-frame relative error:", worst, flush=True)
     for k, e in worst.items():
         assert e < 1e-5, (k, e)
+
+
+def _rb_mix():
+    # sizes from one strip to past the wavefront's 1026 limit; one iteration count
+    return [scenes.stam_scene(N, scenes.derive_seed(23, N), iters=7, frames=2, n_rects=k % 4,
+                              viscosity=10 ** (-3 - k % 3), diffusion=10 ** (-5 + k % 2))
+            for k, N in enumerate((30, 150, 64, 1100, 257, 8))]
+
+
+def test_redblack_batch_equals_solo_and_oracle():
+    """A red-black batch runs every frame kernel over all its scenes; each
+    scene's result equals its solo run and the red-black oracle bit for bit,
+    for every band setting the autotuner may pick."""
+    _case("rb batch")
+    mix = _rb_mix()
+    want = {}
+    for k, sc in enumerate(mix):
+        s = gpu_scene(sc, _abi.GS_REDBLACK)
+        s.step(sc["frames"])
+        s.sync()
+        want[k] = download_all(s)
+        s.close()
+        if sc["size"] <= 152:
+            assert_state_equal(want[k], run_oracle(sc, order=oracle.RB), f"rb solo N={sc['size'] - 2}")
+    for tile, tsw in ((0, 5), (2, 8), (1, 2)):
+        _case(f"rb batch band {tile}:{tsw}")
+        hs = [gpu_scene(sc, _abi.GS_REDBLACK) for sc in mix]
+        b = _abi.Batch(hs)
+        b.set_band(tile, tsw)
+        b.step(mix[0]["frames"])
+        b.sync()
+        b.close()
+        for k, (sc, h) in enumerate(zip(mix, hs)):
+            got = download_all(h)
+            for name in FIELD_NAMES:
+                assert np.array_equal(got[name].view(np.uint32), want[k][name].view(np.uint32)), \
+                    (tile, tsw, sc["size"], name)
+            h.close()
+
+
+def test_redblack_batch_rejects_mixed_modes_and_counts():
+    _case("rb batch errors")
+    a = gpu_scene(scenes.stam_scene(40, 1, iters=4, frames=1), _abi.GS_REDBLACK)
+    b = gpu_scene(scenes.stam_scene(40, 2, iters=4, frames=1), _abi.GS_WAVEFRONT)
+    c = gpu_scene(scenes.stam_scene(40, 3, iters=5, frames=1), _abi.GS_REDBLACK)
+    with pytest.raises(_abi.EqfError):
+        _abi.Batch([a, b])
+    with pytest.raises(_abi.EqfError):
+        _abi.Batch([a, c])
+    bt = _abi.Batch([a])
+    with pytest.raises(_abi.EqfError):
+        bt.set_tuning(8, 0)  # wavefront knob
+    with pytest.raises(_abi.EqfError):
+        bt.set_band(0, 3)
+    bt.close()
