@@ -101,6 +101,66 @@ def bench_batch(scs, steps, warmup, tuning=None):
     return wall, ev_ms, {"max_waves": w, "order": "largest first" if o == 0 else "given", "trial_ms": timings}
 
 
+def bench_batch_redblack(scs, steps, warmup, check_frames=2):
+    """The same mix in red-black mode: every frame kernel covers all 512
+    scenes (band passes over all scenes' tiles with register temporal
+    blocking), band shape and sweeps per launch autotuned; accuracy vs the f64
+    red-black oracle on one scene per size, over the first frames."""
+    import oracle
+    from equilibrium_amd import tune
+
+    def mk(sc):
+        h = _abi.Scene(scenes.scene_params(sc, _abi.GS_REDBLACK))
+        scenes.apply_to_scene(h, sc)
+        return h
+
+    hs = [mk(sc) for sc in scs]
+    for h in hs:
+        h.sync()
+    tile, tsw, timings = tune.tune_batch_band(hs)
+    log(f"red-black batch: autotuned band {tune.TILES[tile]} x {tsw} sweeps/launch: {timings}")
+    b = _abi.Batch(hs)
+    b.set_band(tile, tsw)
+    if warmup:
+        b.step(warmup)
+        b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    ev_ms = b.last_ms()
+    b.close()
+    for h in hs:
+        h.close()
+    cells = sum((sc["size"] - 2) ** 2 for sc in scs)
+    res = {"value": cells * steps / wall, "unit": "cell-updates/s", "ms_per_frame": wall * 1e3 / steps,
+           "launch_ms_per_frame": ev_ms / steps, "band_waves_rows": list(tune.TILES[tile]),
+           "sweeps_per_launch": tsw, "roofline_alg_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
+               HBM_PEAK_GBS * 1e9)}
+    if check_frames > 0:
+        errs = {}
+        for N in sorted({sc["size"] - 2 for sc in scs}):
+            sc = [x for x in scs if x["size"] - 2 == N][0]
+            g = mk(sc)
+            g.step(check_frames)
+            g.sync()
+            st = oracle.State.new(sc["size"]).astype(np.float64)
+            for r in sc["rects"]:
+                st.fill_rect(*r)
+            st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], check_frames, order=oracle.RB,
+                   src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+            e = {}
+            for name, fid in (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY)):
+                got = g.download(fid).astype(np.float64)
+                want = st.f[name]
+                e[name] = float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))
+            g.close()
+            errs[f"N={N}"] = e
+        res["vs_fp64"] = {"frames": check_frames, "rel_norm": errs,
+                          "worst_rel_norm": max(max(v.values()) for v in errs.values())}
+    return res
+
+
 def bench_single_dropin(scs, frames):
     """The plain drop-in: scenes advanced one at a time with eqf_step."""
     hs = make_scenes(scs)
@@ -340,7 +400,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="headline batch only (for profiling)")
-    ap.add_argument("--legs", type=str, default="batch,single,n1024,rb4096,snap,serve,cpu",
+    ap.add_argument("--legs", type=str, default="batch,rbmix,single,n1024,rb4096,snap,serve,cpu",
                     help="comma list of legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration for rocprofv3 --pmc)")
@@ -426,6 +486,11 @@ def main():
                 "launch_ms": launch_ms,
             },
         })
+
+    if "rbmix" in legs:
+        # the batched mix in red-black mode (<= 1e-5 vs FP64 instead of bit-exact)
+        out["batched_mix_redblack"] = bench_batch_redblack(mix, args.steps, args.warmup)
+        log(f"red-black batch: {out['batched_mix_redblack']}")
 
     if "single" in legs:
         # single-grid drop-in on a stratified sample of the same mix
