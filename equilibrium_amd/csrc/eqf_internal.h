@@ -15,6 +15,7 @@ struct eqf_ctx {
 
 struct eqf_scene {
   eqf_ctx* ctx = nullptr;
+  uint64_t uid = 0;  // unique per scene object of the process (never reused)
   eqf_params p{};
   int S = 0, N = 0, ns = 0, P = 0;
   int layout = eqf::L_CYCLIC;
@@ -77,4 +78,5 @@ void* rb_batch_create(const std::vector<eqf_scene*>& scenes, hipStream_t st, std
 void rb_batch_destroy(void* g);
 int rb_batch_set_tiling(void* g, int tile, int tsweeps);
 int rb_batch_step(void* g, int64_t nframes, std::string& err);
+void rb_batch_set_scenes(void* g, const std::vector<eqf_scene*>& scenes);
 }  // namespace eqf

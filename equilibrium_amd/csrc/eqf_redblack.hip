@@ -551,7 +551,7 @@ static uint64_t group_key(const RbGroup& g) {
   auto mix = [&key](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
   for (const eqf_scene* s : g.scenes) {
     const SceneDesc& d = s->desc;
-    mix((uint64_t)(uintptr_t)s);
+    mix(s->uid);
     mix((uint64_t)d.diff_iters);
     mix((uint64_t)d.press_iters);
     mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_visc));
@@ -563,8 +563,8 @@ static uint64_t group_key(const RbGroup& g) {
   return key;
 }
 
-// (Re)writes the group's tables from its scenes' descriptors.
-static int group_tables(RbGroup& g, std::string& err) {
+// (Re)writes the group's tables from its scenes' descriptors, on `st`.
+static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
   const int n = (int)g.scenes.size();
   std::vector<RbScene> scs(n);
   std::vector<int> tstart(n + 1, 0), rstart(n + 1, 0), bpp(n + 1, 0), brc(n + 1, 0), bp(n + 1, 0);
@@ -594,16 +594,17 @@ static int group_tables(RbGroup& g, std::string& err) {
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t b_scs = up(sizeof(RbScene) * n), b_i = up(sizeof(int) * (n + 1)), b_c = up(sizeof(int64_t) * (n + 1));
   const size_t bytes = b_scs + 5 * b_i + b_c;
-  if (bytes > g.tab_bytes) {
+  if (bytes > g.tab_bytes) {  // with headroom: a server's group changes size often
     (void)hipFree(g.d_tab);
     g.d_tab = nullptr;
     g.tab_bytes = 0;
-    hipError_t e = hipMalloc(&g.d_tab, bytes);
+    const size_t cap = std::max(bytes, (size_t)4096) * 2;
+    hipError_t e = hipMalloc(&g.d_tab, cap);
     if (e != hipSuccess) {
       err = std::string("red-black tables: ") + hipGetErrorString(e);
       return EQF_ENOMEM;
     }
-    g.tab_bytes = bytes;
+    g.tab_bytes = cap;
   }
   std::vector<char> h(bytes, 0);
   char* base = h.data();
@@ -619,7 +620,9 @@ static int group_tables(RbGroup& g, std::string& err) {
   g.d_brc = (int*)(d + b_scs + 3 * b_i);
   g.d_bp = (int*)(d + b_scs + 4 * b_i);
   g.d_cstart = (int64_t*)(d + b_scs + 5 * b_i);
-  hipError_t e = hipMemcpy(g.d_tab, h.data(), bytes, hipMemcpyHostToDevice);
+  // behind the frames already queued on `st` (they read the old tables)
+  hipError_t e = hipMemcpyAsync(g.d_tab, h.data(), bytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     err = std::string("red-black tables: ") + hipGetErrorString(e);
     return EQF_EDEVICE;
@@ -734,7 +737,7 @@ static void group_release(RbGroup& g) {
 // captured once as a graph (rebuilt when the scenes' parameters, lists or
 // the tiling change) and replayed.
 static int group_step(RbGroup& g, hipStream_t st, int64_t nframes, std::string& err) {
-  if (nframes <= 0) return EQF_OK;
+  if (nframes <= 0 || g.scenes.empty()) return EQF_OK;
   g.diff_iters = g.scenes[0]->desc.diff_iters;
   g.press_iters = g.scenes[0]->desc.press_iters;
   for (const eqf_scene* s : g.scenes)
@@ -744,7 +747,7 @@ static int group_step(RbGroup& g, hipStream_t st, int64_t nframes, std::string& 
     }
   const uint64_t key = group_key(g);
   if (!g.d_tab || key != g.key || g.tab_tile != g.tile || g.tab_T != g.tsweeps) {
-    int rc = group_tables(g, err);
+    int rc = group_tables(g, st, err);
     if (rc) return rc;
     if (g.graph) (void)hipGraphExecDestroy(g.graph);
     g.graph = nullptr;
@@ -857,6 +860,10 @@ int rb_batch_set_tiling(void* p, int tile, int tsweeps) {
 int rb_batch_step(void* p, int64_t nframes, std::string& err) {
   auto* g = static_cast<RbGroup*>(p);
   return group_step(*g, g->stream, nframes, err);
+}
+void rb_batch_set_scenes(void* p, const std::vector<eqf_scene*>& scenes) {
+  auto* g = static_cast<RbGroup*>(p);
+  g->scenes = scenes;  // a new membership changes the key: tables and graph are rebuilt
 }
 
 }  // namespace eqf

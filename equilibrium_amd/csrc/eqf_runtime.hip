@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -211,6 +212,8 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
   HIPCHK(hipSetDevice(ctx->device));
   auto* s = new (std::nothrow) eqf_scene;
   if (!s) return fail(EQF_ENOMEM, "host allocation failed");
+  static std::atomic<uint64_t> next_uid{1};
+  s->uid = next_uid++;
   s->ctx = ctx;
   s->p = *p;
   s->S = (int)p->size;

@@ -3,8 +3,8 @@
 // A server thread owns every admitted scene.  Each tick it launches ONE fused
 // frame kernel (eqf_frames_cyclic, grid = resident wavefront-mode scenes) that
 // advances every such scene by one frame (fluid.rs:437-524); red-black scenes
-// (any size, for grids past the wavefront's 1026 limit) advance by one
-// replay of their frame graph each, on a second stream beside it.  While the
+// (any size, for grids past the wavefront's 1026 limit) advance beside it by
+// one frame of their launch group (every kernel covers all of them).  While the
 // tick runs,
 // the thread admits queued requests whose device memory fits the budget:
 // it allocates them, writes Fluid::new's state, marks obstacles and uploads
@@ -19,6 +19,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -76,6 +77,9 @@ struct eqf_server {
   static constexpr int kRbStreams = 2;
   hipStream_t rstream[kRbStreams] = {nullptr, nullptr};
   int rb_next = 0;
+  // launch groups of the resident red-black scenes, one per (diffusion,
+  // pressure) iteration counts: every frame kernel covers all scenes of a group
+  std::map<std::pair<int, int>, void*> rb_groups;
   SceneDesc* d_descs = nullptr;
   int desc_cap = 0;
   Clock::time_point t0;
@@ -185,7 +189,8 @@ static void server_loop(eqf_server* sv) {
       act = sv->active;
     }
     // Launch one frame of every resident scene: the wavefront-mode scenes in
-    // one fused launch (largest first), the red-black ones by their graphs.
+    // one fused launch (largest first), the red-black ones by their launch
+    // groups (one per pair of iteration counts, one graph replay each).
     bool launched = false, launched_rb = false;
     double t_launch = sv->now_ms();
     std::vector<Req*> wf, rbs;
@@ -222,15 +227,34 @@ static void server_loop(eqf_server* sv) {
         launched = true;
       }
     }
-    for (Req* r : rbs) {
-      std::string err;
-      if (rb_step(r->scene, 1, err) != EQF_OK) {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->thread_error = err;
-        r->state = EQF_REQ_FAILED;
-        r->cancel = true;  // dropped at the next tick
-      } else {
-        launched_rb = true;
+    {
+      std::map<std::pair<int, int>, std::vector<Req*>> by_iters;
+      for (Req* r : rbs) by_iters[{r->scene->desc.diff_iters, r->scene->desc.press_iters}].push_back(r);
+      for (auto& [key, reqs] : by_iters) {
+        std::stable_sort(reqs.begin(), reqs.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
+        std::vector<eqf_scene*> members;
+        for (Req* r : reqs) members.push_back(r->scene);
+        std::string err;
+        void*& g = sv->rb_groups[key];
+        if (!g) {
+          g = rb_batch_create(members, sv->rstream[sv->rb_next], err);
+          sv->rb_next = (sv->rb_next + 1) % eqf_server::kRbStreams;
+        }
+        int rc = g ? EQF_OK : EQF_EINVAL;
+        if (g) {
+          rb_batch_set_scenes(g, members);
+          rc = rb_batch_step(g, 1, err);
+        }
+        if (rc != EQF_OK) {
+          std::lock_guard<std::mutex> lk(sv->mu);
+          sv->thread_error = err;
+          for (Req* r : reqs) {
+            r->state = EQF_REQ_FAILED;
+            r->cancel = true;  // dropped at the next tick
+          }
+        } else {
+          launched_rb = true;
+        }
       }
     }
     auto tick_running = [&] {
@@ -510,6 +534,10 @@ extern "C" int eqf_server_destroy(eqf_server* sv) {
   }
   if (sv->th.joinable()) sv->th.join();
   (void)hipStreamSynchronize(sv->stream);
+  for (auto& kv : sv->rb_groups) {
+    for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
+    rb_batch_destroy(kv.second);
+  }
   for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
   (void)hipFree(sv->d_descs);
   (void)hipStreamDestroy(sv->stream);

@@ -98,15 +98,16 @@ def test_server_rejects_oversized_and_cancels():
 
 
 def test_server_red_black_requests_beside_wavefront():
-    """Red-black requests (one past the wavefront's size limit) are served on
-    their own stream beside the fused launch; each result equals its solo
+    """Red-black requests (one past the wavefront's size limit) are served
+    beside the fused launch, as launch groups (one per pair of iteration
+    counts, members leaving as they finish); each result equals its solo
     red-black run bit for bit, and a small one equals the red-black oracle."""
     import oracle
 
     _case("server red-black mix")
     wf = _mix()[:4]
-    rbs = [scenes.stam_scene(N, scenes.derive_seed(31, N), iters=6, frames=fr, n_rects=2)
-           for N, fr in ((150, 4), (1100, 2), (300, 3))]
+    rbs = [scenes.stam_scene(N, scenes.derive_seed(31, N), iters=it, frames=fr, n_rects=2)
+           for N, fr, it in ((150, 4, 6), (1100, 2, 6), (300, 3, 6), (64, 5, 9), (200, 2, 9))]
     srv = _abi.Server(hbm_budget_bytes=8 << 30)
     tw = [srv.submit(sc) for sc in wf]
     tr = [srv.submit(sc, gs_mode=_abi.GS_REDBLACK) for sc in rbs]
