@@ -411,31 +411,62 @@ __global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ sc
 }
 
 // advect, pass B (fluid.rs:398-430) for one or two fields sharing u, v
-// (o1_role ROLE_NONE: one field).
-__global__ void k_adv(const RbScene* __restrict__ scs, const int* __restrict__ rstart, int nsc, int o0_role,
-                      int i0_role, int o1_role, int i1_role, int u_role, int v_role) {
+// (o1_role ROLE_NONE: one field).  Each thread takes K columns of one row,
+// 256 apart: the row's break column and the velocities of all K cells are
+// loaded together, then every gather, then the stores, so a wave keeps up to
+// 10K loads in flight.  Cells right of the break keep the buffer's content;
+// their (clamped, in-grid) gathers are computed and dropped.
+constexpr int ADV_K = 4;
+__global__ void __launch_bounds__(256) k_adv(const RbScene* __restrict__ scs, const int* __restrict__ rstart,
+                                             int nsc, int o0_role, int i0_role, int o1_role, int i1_role,
+                                             int u_role, int v_role) {
+  constexpr int K = ADV_K;
   const int gr = blockIdx.y;
   const int js = find_scene(rstart, nsc, gr);
   const RbScene& sc = scs[js];
   const int S = sc.d.S, N = sc.d.N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = gr - rstart[js] + 1;
-  if (i > N) return;
-  const int tr = sc.trig[j];
-  if (i > tr) return;  // right of the break: buffer keeps its content
+  const int i_first = blockIdx.x * (256 * K) + threadIdx.x + 1, j = gr - rstart[js] + 1;
+  if (i_first > N) return;
   float* __restrict__ o0 = role_ptr(sc, o0_role);
   float* __restrict__ o1 = o1_role == ROLE_NONE ? nullptr : role_ptr(sc, o1_role);
-  const int q = j * S + i;
-  if (i == tr && i == 1) {  // copies the ring cell d(0, j)
-    o0[q] = o0[q - 1];
-    if (o1) o1[q] = o1[q - 1];
-    return;
-  }
-  const int c = i == tr ? i - 1 : i;  // the trigger cell copies its left neighbour's value
+  const float* __restrict__ d0 = role_ptr(sc, i0_role);
+  const float* __restrict__ d1 = o1 ? role_ptr(sc, i1_role) : d0;
   const float* __restrict__ u = role_ptr(sc, u_role);
   const float* __restrict__ v = role_ptr(sc, v_role);
-  const Trace r = back_trace(sc.d, c, j, u[j * S + c], v[j * S + c]);
-  o0[q] = gather(role_ptr(sc, i0_role), r, S);
-  if (o1) o1[q] = gather(role_ptr(sc, i1_role), r, S);
+  const int row = j * S;
+  const int tr = sc.trig[j];
+  float uu[K], vv[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {  // columns past N read column N (unused)
+    const int i = min(i_first + k * 256, N);
+    uu[k] = u[row + i];
+    vv[k] = v[row + i];
+  }
+  float r0[K], r1[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = i_first + k * 256;
+    int c = i;
+    if (i == tr && i > 1) {  // the trigger cell copies its left neighbour's value (at most one per row)
+      c = i - 1;
+      uu[k] = u[row + c];
+      vv[k] = v[row + c];
+    }
+    const Trace t = back_trace(sc.d, c, j, uu[k], vv[k]);
+    r0[k] = gather(d0, t, S);
+    r1[k] = gather(d1, t, S);
+    if (i == tr && i == 1) {  // trigger in column 1: copies the ring cell d(0, j)
+      r0[k] = o0[row];
+      if (o1) r1[k] = o1[row];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = i_first + k * 256;
+    if (i > N || i > tr) continue;  // right of the break: buffer keeps its content
+    o0[row + i] = r0[k];
+    if (o1) o1[row + i] = r1[k];
+  }
 }
 
 // set_boundaries as lists (built from the mask on the device): entry e of a
@@ -695,6 +726,7 @@ static void g_bnd(hipStream_t st, const RbGroup& g, int mode0, int x0, int mode1
 static void g_frame(hipStream_t st, const RbGroup& g) {
   using rb::ROLE_NONE;
   const dim3 rows((unsigned)((g.maxN + 255) / 256), (unsigned)g.rows);
+  const dim3 rows_adv((unsigned)((g.maxN + 256 * rb::ADV_K - 1) / (256 * rb::ADV_K)), (unsigned)g.rows);
   if (g.any_src)
     hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)((g.cells + 255) / 256)), dim3(256), 0, st, g.d_scs, g.d_cstart,
                        g.n);
@@ -708,7 +740,8 @@ static void g_frame(hipStream_t st, const RbGroup& g) {
   g_bnd(st, g, M_ROW, F_U0, M_COL, F_V0);
   // advect U from U0 and V from V0 with (U0, V0), fluid.rs:469-489
   hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0);
-  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_U0, F_V, F_V0, F_U0, F_V0);
+  hipLaunchKernelGGL(rb::k_adv, rows_adv, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_U0, F_V, F_V0, F_U0,
+                     F_V0);
   g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // project(U, V, p=U0, div=V0), fluid.rs:491
   hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V, F_V0, F_U0);
@@ -719,8 +752,8 @@ static void g_frame(hipStream_t st, const RbGroup& g) {
   // density, fluid.rs:501-523
   g_lin_solve(st, g, M_PASSIVE, F_SC, F_D, 1, g.diff_iters);
   hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V);
-  hipLaunchKernelGGL(rb::k_adv, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_D, F_SC, ROLE_NONE, ROLE_NONE,
-                     F_U, F_V);
+  hipLaunchKernelGGL(rb::k_adv, rows_adv, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_D, F_SC, ROLE_NONE,
+                     ROLE_NONE, F_U, F_V);
   g_bnd(st, g, M_PASSIVE, F_D, M_PASSIVE, ROLE_NONE);
   g_copy(st, g, F_SC, F_D);
 }
