@@ -68,9 +68,12 @@ enum BandSel : uint32_t {
 };
 
 // One scene of a red-black launch group: a single scene, or every scene of
-// a batch.  Each kernel below covers all scenes of its group in one launch;
-// a workgroup finds its scene by binary search over a prefix table of the
-// launch's work units (band tiles, rows, cells or list entries per scene).
+// a batch.  Each kernel below covers all scenes of its group in one launch.
+// A workgroup finds its scene and its place in it with one load from a map
+// of the launch's work units (band tile, interior row, block of CELL_BLOCK
+// interior cells or of CELL_BLOCK cells ring included -> {scene, index in
+// the scene}); the set_boundaries list kernel, whose threads straddle
+// scenes, searches a prefix table instead.
 struct RbScene {
   SceneDesc d;  // the scene's descriptor (a copy): fields, sizes, coefficients, rule bytes, lists
   float* tmp;   // its ping-pong field
@@ -79,6 +82,36 @@ struct RbScene {
 // Field roles in kernel arguments: F_* (eqf_layout.h), the ping-pong field, or none.
 constexpr int ROLE_TMP = NFIELDS;
 constexpr int ROLE_NONE = -1;
+// cells per workgroup of the whole-field kernels (256 threads x 4 cells)
+constexpr int CELL_K = 4;
+constexpr int CELL_BLOCK = 256 * CELL_K;
+
+// The CELL_K interior cells of this thread in interior block `blk` of a
+// scene: flat indices blk*CELL_BLOCK + k*256 + threadIdx.x over the N x N
+// interior in row-major order, as 1-based (i, j).  Indices past N*N are
+// flagged !ok and clamped to cell (N, N), so their loads stay in the grid.
+__device__ __forceinline__ void interior_cells(int blk, int N, int (&ci)[CELL_K], int (&cj)[CELL_K],
+                                               bool (&ok)[CELL_K]) {
+  const int64_t idx0 = (int64_t)blk * CELL_BLOCK, nn = (int64_t)N * N;
+  const int j0 = (int)(idx0 / N), i0 = (int)(idx0 - (int64_t)j0 * N);
+  const float inv = 1.0f / (float)N;
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) {
+    const int off = i0 + k * 256 + (int)threadIdx.x;  // < N + CELL_BLOCK: exact in f32
+    int q = (int)((float)off * inv);
+    int r = off - q * N;
+    if (r < 0) {  // the rounded reciprocal is off by at most one row
+      --q;
+      r += N;
+    } else if (r >= N) {
+      ++q;
+      r -= N;
+    }
+    ok[k] = idx0 + k * 256 + threadIdx.x < nn;
+    ci[k] = ok[k] ? r + 1 : N;
+    cj[k] = ok[k] ? j0 + q + 1 : N;
+  }
+}
 __device__ __forceinline__ float* role_ptr(const RbScene& s, int role) {
   return role == ROLE_TMP ? s.tmp : s.d.fld + (int64_t)role * s.d.fstride;
 }
@@ -116,9 +149,9 @@ __device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, fl
 // buffer and the tile is written to the other (ping-pong).  Corners are set
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
-__global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__ scs, const int* __restrict__ tstart,
-                                                     int nsc, int nt, int src_role, int dst_role, int x0_role,
-                                                     int coef, int nsweeps) {
+__global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
+                                                     int nt, int src_role, int dst_role, int x0_role, int coef,
+                                                     int nsweeps) {
   constexpr int H = 3 * T, W = 128, RH = NW * R;
   constexpr int TX = W - 2 * H, TY = RH - 2 * H;
   static_assert(TX > 0 && TY > 0 && TX % 2 == 0 && TY % 2 == 0 && R % 2 == 0, "band geometry");
@@ -132,16 +165,17 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__
   // instead, so tiles that share halo columns run together on one L2.
   int t;
   {
-    const int L = blockIdx.x;  // nt = tstart[nsc] workgroups
+    const int L = blockIdx.x;  // nt workgroups, one per tile
     const int per = nt >> 3, rem = nt & 7, xcd = L & 7;
     t = __builtin_amdgcn_readfirstlane(xcd * per + min(xcd, rem) + (L >> 3));
   }
-  const int js = __builtin_amdgcn_readfirstlane(find_scene(tstart, nsc, t));
+  const int2 tm = tmap[t];
+  const int js = __builtin_amdgcn_readfirstlane(tm.x), tl = __builtin_amdgcn_readfirstlane(tm.y);
   const RbScene& sc = scs[js];
   const SceneDesc& d = sc.d;
   const int S = d.S, N = d.N;
   const int tiles_x = (S + TX - 1) / TX;
-  const int bx = (t - tstart[js]) % tiles_x, by = (t - tstart[js]) / tiles_x;
+  const int bx = tl % tiles_x, by = tl / tiles_x;
   const float* __restrict__ src = role_ptr(sc, src_role);
   float* __restrict__ dst = role_ptr(sc, dst_role);
   const float* __restrict__ x0g = role_ptr(sc, x0_role);
@@ -303,39 +337,71 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__
   }
 }
 
-// project, first half (fluid.rs:339-349).  Grid: (column blocks of 256,
-// rows of all scenes); rstart = prefix of the scenes' interior rows.
-__global__ void k_div(const RbScene* __restrict__ scs, const int* __restrict__ rstart, int nsc, int u_role,
-                      int v_role, int div_role, int p_role) {
-  const int gr = blockIdx.y;
-  const int js = find_scene(rstart, nsc, gr);
-  const RbScene& sc = scs[js];
+// project, first half (fluid.rs:339-349).  imap: workgroup -> {scene, block
+// of CELL_BLOCK interior cells}; every load of a thread's cells is issued
+// before its stores.
+__global__ void __launch_bounds__(256) k_div(const RbScene* __restrict__ scs, const int2* __restrict__ imap,
+                                             int u_role, int v_role, int div_role, int p_role) {
+  const int2 im = imap[blockIdx.x];
+  const RbScene& sc = scs[im.x];
   const int S = sc.d.S, N = sc.d.N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = gr - rstart[js] + 1;
-  if (i > N) return;
+  int ci[CELL_K], cj[CELL_K];
+  bool ok[CELL_K];
+  interior_cells(im.y, N, ci, cj, ok);
   const float* __restrict__ u = role_ptr(sc, u_role);
   const float* __restrict__ v = role_ptr(sc, v_role);
-  const int q = j * S + i;
-  role_ptr(sc, div_role)[q] = -0.5f * (u[q + 1] - u[q - 1] + v[q + S] - v[q - S]) / sc.d.sf;
-  role_ptr(sc, p_role)[q] = 0.0f;
+  float ur[CELL_K], ul[CELL_K], vd[CELL_K], vu[CELL_K];
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) {
+    const int q = cj[k] * S + ci[k];
+    ur[k] = u[q + 1];
+    ul[k] = u[q - 1];
+    vd[k] = v[q + S];
+    vu[k] = v[q - S];
+  }
+  float* __restrict__ dv = role_ptr(sc, div_role);
+  float* __restrict__ p = role_ptr(sc, p_role);
+  const float sf = sc.d.sf;
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) {
+    if (!ok[k]) continue;
+    const int q = cj[k] * S + ci[k];
+    dv[q] = -0.5f * (ur[k] - ul[k] + vd[k] - vu[k]) / sf;
+    p[q] = 0.0f;
+  }
 }
 
-// project, second half (fluid.rs:364-371)
-__global__ void k_grad(const RbScene* __restrict__ scs, const int* __restrict__ rstart, int nsc, int u_role,
-                       int v_role, int p_role) {
-  const int gr = blockIdx.y;
-  const int js = find_scene(rstart, nsc, gr);
-  const RbScene& sc = scs[js];
+// project, second half (fluid.rs:364-371); imap as for k_div.
+__global__ void __launch_bounds__(256) k_grad(const RbScene* __restrict__ scs, const int2* __restrict__ imap,
+                                              int u_role, int v_role, int p_role) {
+  const int2 im = imap[blockIdx.x];
+  const RbScene& sc = scs[im.x];
   const int S = sc.d.S, N = sc.d.N;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1, j = gr - rstart[js] + 1;
-  if (i > N) return;
+  int ci[CELL_K], cj[CELL_K];
+  bool ok[CELL_K];
+  interior_cells(im.y, N, ci, cj, ok);
   float* __restrict__ u = role_ptr(sc, u_role);
   float* __restrict__ v = role_ptr(sc, v_role);
   const float* __restrict__ p = role_ptr(sc, p_role);
+  float uq[CELL_K], vq[CELL_K], pr[CELL_K], pl[CELL_K], pd[CELL_K], pu[CELL_K];
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) {
+    const int q = cj[k] * S + ci[k];
+    uq[k] = u[q];
+    vq[k] = v[q];
+    pr[k] = p[q + 1];
+    pl[k] = p[q - 1];
+    pd[k] = p[q + S];
+    pu[k] = p[q - S];
+  }
   const float sf = sc.d.sf;
-  const int q = j * S + i;
-  u[q] = u[q] - 0.5f * (p[q + 1] - p[q - 1]) * sf;
-  v[q] = v[q] - 0.5f * (p[q + S] - p[q - S]) * sf;
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) {
+    if (!ok[k]) continue;
+    const int q = cj[k] * S + ci[k];
+    u[q] = uq[k] - 0.5f * (pr[k] - pl[k]) * sf;
+    v[q] = vq[k] - 0.5f * (pd[k] - pu[k]) * sf;
+  }
 }
 
 struct Trace {
@@ -370,14 +436,13 @@ __device__ __forceinline__ float gather(const float* __restrict__ d0, const Trac
 // edge (the `break` of fluid.rs:420-423); N+1 when none.  One workgroup per
 // row of any scene; each thread tests 8 columns per round with all 16 loads
 // in flight.
-__global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ scs, const int* __restrict__ rstart,
-                                                  int nsc, int u_role, int v_role) {
-  const int gr = blockIdx.x;
-  const int js = find_scene(rstart, nsc, gr);
-  const RbScene& sc = scs[js];
+__global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ scs, const int2* __restrict__ rmap,
+                                                  int u_role, int v_role) {
+  const int2 rm = rmap[blockIdx.x];
+  const RbScene& sc = scs[rm.x];
   const SceneDesc& d = sc.d;
   const int S = d.S, N = d.N;
-  const int j = gr - rstart[js] + 1;
+  const int j = rm.y;
   const float* __restrict__ u = role_ptr(sc, u_role);
   const float* __restrict__ v = role_ptr(sc, v_role);
   const int64_t row = (int64_t)j * S;
@@ -411,43 +476,40 @@ __global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ sc
 }
 
 // advect, pass B (fluid.rs:398-430) for one or two fields sharing u, v
-// (o1_role ROLE_NONE: one field).  Each thread takes K columns of one row,
-// 256 apart: the row's break column and the velocities of all K cells are
-// loaded together, then every gather, then the stores, so a wave keeps up to
-// 10K loads in flight.  Cells right of the break keep the buffer's content;
+// (o1_role ROLE_NONE: one field); imap as for k_div.  The break columns and
+// velocities of a thread's cells are loaded together, then every gather, then
+// the stores.  Cells right of their row's break keep the buffer's content;
 // their (clamped, in-grid) gathers are computed and dropped.
-constexpr int ADV_K = 4;
-__global__ void __launch_bounds__(256) k_adv(const RbScene* __restrict__ scs, const int* __restrict__ rstart,
-                                             int nsc, int o0_role, int i0_role, int o1_role, int i1_role,
-                                             int u_role, int v_role) {
-  constexpr int K = ADV_K;
-  const int gr = blockIdx.y;
-  const int js = find_scene(rstart, nsc, gr);
-  const RbScene& sc = scs[js];
+__global__ void __launch_bounds__(256) k_adv(const RbScene* __restrict__ scs, const int2* __restrict__ imap,
+                                             int o0_role, int i0_role, int o1_role, int i1_role, int u_role,
+                                             int v_role) {
+  const int2 im = imap[blockIdx.x];
+  const RbScene& sc = scs[im.x];
   const int S = sc.d.S, N = sc.d.N;
-  const int i_first = blockIdx.x * (256 * K) + threadIdx.x + 1, j = gr - rstart[js] + 1;
-  if (i_first > N) return;
+  int ci[CELL_K], cj[CELL_K];
+  bool ok[CELL_K];
+  interior_cells(im.y, N, ci, cj, ok);
   float* __restrict__ o0 = role_ptr(sc, o0_role);
   float* __restrict__ o1 = o1_role == ROLE_NONE ? nullptr : role_ptr(sc, o1_role);
   const float* __restrict__ d0 = role_ptr(sc, i0_role);
   const float* __restrict__ d1 = o1 ? role_ptr(sc, i1_role) : d0;
   const float* __restrict__ u = role_ptr(sc, u_role);
   const float* __restrict__ v = role_ptr(sc, v_role);
-  const int row = j * S;
-  const int tr = sc.trig[j];
-  float uu[K], vv[K];
+  int tr[CELL_K];
+  float uu[CELL_K], vv[CELL_K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {  // columns past N read column N (unused)
-    const int i = min(i_first + k * 256, N);
-    uu[k] = u[row + i];
-    vv[k] = v[row + i];
+  for (int k = 0; k < CELL_K; ++k) {
+    const int q = cj[k] * S + ci[k];
+    tr[k] = sc.trig[cj[k]];
+    uu[k] = u[q];
+    vv[k] = v[q];
   }
-  float r0[K], r1[K];
+  float r0[CELL_K], r1[CELL_K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int i = i_first + k * 256;
+  for (int k = 0; k < CELL_K; ++k) {
+    const int i = ci[k], j = cj[k], row = j * S;
     int c = i;
-    if (i == tr && i > 1) {  // the trigger cell copies its left neighbour's value (at most one per row)
+    if (i == tr[k] && i > 1) {  // the trigger cell copies its left neighbour's value (at most one per row)
       c = i - 1;
       uu[k] = u[row + c];
       vv[k] = v[row + c];
@@ -455,17 +517,17 @@ __global__ void __launch_bounds__(256) k_adv(const RbScene* __restrict__ scs, co
     const Trace t = back_trace(sc.d, c, j, uu[k], vv[k]);
     r0[k] = gather(d0, t, S);
     r1[k] = gather(d1, t, S);
-    if (i == tr && i == 1) {  // trigger in column 1: copies the ring cell d(0, j)
+    if (i == tr[k] && i == 1) {  // trigger in column 1: copies the ring cell d(0, j)
       r0[k] = o0[row];
       if (o1) r1[k] = o1[row];
     }
   }
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int i = i_first + k * 256;
-    if (i > N || i > tr) continue;  // right of the break: buffer keeps its content
-    o0[row + i] = r0[k];
-    if (o1) o1[row + i] = r1[k];
+  for (int k = 0; k < CELL_K; ++k) {
+    if (!ok[k] || ci[k] > tr[k]) continue;  // right of the break: buffer keeps its content
+    const int q = cj[k] * S + ci[k];
+    o0[q] = r0[k];
+    if (o1) o1[q] = r1[k];
   }
 }
 
@@ -504,33 +566,64 @@ __global__ void k_corners(const RbScene* __restrict__ scs, int nsc, int role0, i
 }
 
 // Frame sources (only the fields that were set; adding 0 would turn -0 into
-// +0) over every cell of every scene; cstart = prefix of the scenes' S*S.
-__global__ void k_sources(const RbScene* __restrict__ scs, const int64_t* __restrict__ cstart, int nsc) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= cstart[nsc]) return;
-  const int js = find_scene(cstart, nsc, g);
-  const SceneDesc& d = scs[js].d;
+// +0) over every cell of every scene.  cmap: workgroup -> {scene, block of
+// CELL_BLOCK cells}; each thread loads its CELL_K cells (256 apart) before
+// storing any.
+__device__ __forceinline__ void add_field(float* __restrict__ f, const float* __restrict__ a, int64_t e0,
+                                          int64_t cells) {
+  float x[CELL_K], y[CELL_K];
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) {
+    const int64_t e = min(e0 + k * 256, cells - 1);
+    x[k] = f[e];
+    y[k] = a[e];
+  }
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k)
+    if (e0 + k * 256 < cells) f[e0 + k * 256] = x[k] + y[k];
+}
+__global__ void __launch_bounds__(256) k_sources(const RbScene* __restrict__ scs, const int2* __restrict__ cmap) {
+  const int2 cm = cmap[blockIdx.x];
+  const SceneDesc& d = scs[cm.x].d;
   const int m = d.has_src;
   if (!m) return;
-  const int64_t n = d.fstride, e = g - cstart[js];
+  const int64_t n = d.fstride, cells = (int64_t)d.S * d.S;
+  const int64_t e0 = (int64_t)cm.y * CELL_BLOCK + threadIdx.x;
   if (m & 1) {
-    const float a = d.src[e];
-    d.fld[F_D * n + e] += a;
-    d.fld[F_SC * n + e] += a;
+    float a[CELL_K], x[CELL_K], y[CELL_K];
+#pragma unroll
+    for (int k = 0; k < CELL_K; ++k) {
+      const int64_t e = min(e0 + k * 256, cells - 1);
+      a[k] = d.src[e];
+      x[k] = d.fld[F_D * n + e];
+      y[k] = d.fld[F_SC * n + e];
+    }
+#pragma unroll
+    for (int k = 0; k < CELL_K; ++k)
+      if (e0 + k * 256 < cells) {
+        d.fld[F_D * n + e0 + k * 256] = x[k] + a[k];
+        d.fld[F_SC * n + e0 + k * 256] = y[k] + a[k];
+      }
   }
-  if (m & 2) d.fld[F_U * n + e] += d.src[n + e];
-  if (m & 4) d.fld[F_V * n + e] += d.src[2 * n + e];
+  if (m & 2) add_field(d.fld + F_U * n, d.src + n, e0, cells);
+  if (m & 4) add_field(d.fld + F_V * n, d.src + 2 * n, e0, cells);
 }
 
-// dst = src over every cell of every scene.
-__global__ void k_copy(const RbScene* __restrict__ scs, const int64_t* __restrict__ cstart, int nsc, int dst_role,
-                       int src_role) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= cstart[nsc]) return;
-  const int js = find_scene(cstart, nsc, g);
-  const RbScene& sc = scs[js];
-  const int64_t e = g - cstart[js];
-  role_ptr(sc, dst_role)[e] = role_ptr(sc, src_role)[e];
+// dst = src over every cell of every scene (cmap as for k_sources).
+__global__ void __launch_bounds__(256) k_copy(const RbScene* __restrict__ scs, const int2* __restrict__ cmap,
+                                              int dst_role, int src_role) {
+  const int2 cm = cmap[blockIdx.x];
+  const RbScene& sc = scs[cm.x];
+  const int64_t cells = (int64_t)sc.d.S * sc.d.S;
+  const int64_t e0 = (int64_t)cm.y * CELL_BLOCK + threadIdx.x;
+  const float* __restrict__ src = role_ptr(sc, src_role);
+  float* __restrict__ dst = role_ptr(sc, dst_role);
+  float x[CELL_K];
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k) x[k] = src[min(e0 + k * 256, cells - 1)];
+#pragma unroll
+  for (int k = 0; k < CELL_K; ++k)
+    if (e0 + k * 256 < cells) dst[e0 + k * 256] = x[k];
 }
 
 }  // namespace rb
@@ -544,11 +637,12 @@ struct RbGroup {
   hipStream_t stream = nullptr;    // null: each frame runs on scenes[0]'s stream
   void* d_tab = nullptr;
   size_t tab_bytes = 0;
+  std::vector<char> h_tab;  // host image of the tables
   RbScene* d_scs = nullptr;
-  int *d_tstart = nullptr, *d_rstart = nullptr, *d_bpp = nullptr, *d_brc = nullptr, *d_bp = nullptr;
-  int64_t* d_cstart = nullptr;
-  int n = 0, rows = 0, maxN = 0, tiles = 0, bpp = 0, brc = 0, bp = 0;
-  int64_t cells = 0;
+  // work unit -> {scene, index in it}: band tiles, interior rows, interior cell blocks, cell blocks
+  int2 *d_tmap = nullptr, *d_rmap = nullptr, *d_imap = nullptr, *d_cmap = nullptr;
+  int *d_bpp = nullptr, *d_brc = nullptr, *d_bp = nullptr;  // prefixes of list entries
+  int n = 0, rows = 0, maxN = 0, tiles = 0, iblocks = 0, cblocks = 0, bpp = 0, brc = 0, bp = 0;
   bool any_src = false;
   int diff_iters = 0, press_iters = 0;
   int tile = 0, tsweeps = 4;  // band shape index (kBands) and sweeps per launch
@@ -598,33 +692,44 @@ static uint64_t group_key(const RbGroup& g) {
 static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
   const int n = (int)g.scenes.size();
   std::vector<RbScene> scs(n);
-  std::vector<int> tstart(n + 1, 0), rstart(n + 1, 0), bpp(n + 1, 0), brc(n + 1, 0), bp(n + 1, 0);
-  std::vector<int64_t> cstart(n + 1, 0);
+  std::vector<int2> tmap, rmap, imap, cmap;
+  std::vector<int> bpp(n + 1, 0), brc(n + 1, 0), bp(n + 1, 0);
   g.any_src = false;
   g.maxN = 0;
   for (int i = 0; i < n; ++i) {
     eqf_scene* s = g.scenes[i];
     auto* w = static_cast<RBWork*>(s->rb);
     scs[i] = RbScene{s->desc, w->tmp, w->trig};
-    tstart[i + 1] = tstart[i] + band_tiles(s->S, g.tile, g.tsweeps);
-    rstart[i + 1] = rstart[i] + s->N;
-    cstart[i + 1] = cstart[i] + (int64_t)s->S * s->S;
+    const int nt = band_tiles(s->S, g.tile, g.tsweeps);
+    for (int t = 0; t < nt; ++t) tmap.push_back(make_int2(i, t));
+    for (int j = 1; j <= s->N; ++j) rmap.push_back(make_int2(i, j));
+    const int64_t ni = ((int64_t)s->N * s->N + rb::CELL_BLOCK - 1) / rb::CELL_BLOCK;
+    for (int64_t k = 0; k < ni; ++k) imap.push_back(make_int2(i, (int)k));
+    const int64_t nb = ((int64_t)s->S * s->S + rb::CELL_BLOCK - 1) / rb::CELL_BLOCK;
+    for (int64_t k = 0; k < nb; ++k) cmap.push_back(make_int2(i, (int)k));
     bpp[i + 1] = bpp[i] + 2 * s->nbnd[M_PASSIVE];
     brc[i + 1] = brc[i] + s->nbnd[M_ROW] + s->nbnd[M_COL];
     bp[i + 1] = bp[i] + s->nbnd[M_PASSIVE];
     g.any_src = g.any_src || s->desc.has_src;
     g.maxN = std::max(g.maxN, s->N);
   }
+  if (tmap.size() > INT32_MAX / 2 || rmap.size() > INT32_MAX / 2 || cmap.size() > INT32_MAX / 2) {  // imap < cmap
+    err = "red-black launch group too large for one launch";
+    return EQF_EINVAL;
+  }
   g.n = n;
-  g.tiles = tstart[n];
-  g.rows = rstart[n];
-  g.cells = cstart[n];
+  g.tiles = (int)tmap.size();
+  g.rows = (int)rmap.size();
+  g.iblocks = (int)imap.size();
+  g.cblocks = (int)cmap.size();
   g.bpp = bpp[n];
   g.brc = brc[n];
   g.bp = bp[n];
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-  const size_t b_scs = up(sizeof(RbScene) * n), b_i = up(sizeof(int) * (n + 1)), b_c = up(sizeof(int64_t) * (n + 1));
-  const size_t bytes = b_scs + 5 * b_i + b_c;
+  const size_t b_scs = up(sizeof(RbScene) * n), b_i = up(sizeof(int) * (n + 1));
+  const size_t b_t = up(sizeof(int2) * tmap.size()), b_r = up(sizeof(int2) * rmap.size());
+  const size_t b_m = up(sizeof(int2) * imap.size()), b_c = up(sizeof(int2) * cmap.size());
+  const size_t bytes = b_scs + 3 * b_i + b_t + b_r + b_m + b_c;
   if (bytes > g.tab_bytes) {  // with headroom: a server's group changes size often
     (void)hipFree(g.d_tab);
     g.d_tab = nullptr;
@@ -637,22 +742,34 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
     }
     g.tab_bytes = cap;
   }
-  std::vector<char> h(bytes, 0);
-  char* base = h.data();
-  std::memcpy(base, scs.data(), sizeof(RbScene) * n);
-  int* ib[5] = {tstart.data(), rstart.data(), bpp.data(), brc.data(), bp.data()};
-  for (int k = 0; k < 5; ++k) std::memcpy(base + b_scs + k * b_i, ib[k], sizeof(int) * (n + 1));
-  std::memcpy(base + b_scs + 5 * b_i, cstart.data(), sizeof(int64_t) * (n + 1));
+  g.h_tab.assign(bytes, 0);
+  char* base = g.h_tab.data();
+  size_t off = 0;
+  auto put = [&](const void* src, size_t len, size_t slot) {
+    if (len) std::memcpy(base + off, src, len);
+    const size_t at = off;
+    off += slot;
+    return at;
+  };
+  const size_t o_scs = put(scs.data(), sizeof(RbScene) * n, b_scs);
+  const size_t o_bpp = put(bpp.data(), sizeof(int) * (n + 1), b_i);
+  const size_t o_brc = put(brc.data(), sizeof(int) * (n + 1), b_i);
+  const size_t o_bp = put(bp.data(), sizeof(int) * (n + 1), b_i);
+  const size_t o_t = put(tmap.data(), sizeof(int2) * tmap.size(), b_t);
+  const size_t o_r = put(rmap.data(), sizeof(int2) * rmap.size(), b_r);
+  const size_t o_m = put(imap.data(), sizeof(int2) * imap.size(), b_m);
+  const size_t o_c = put(cmap.data(), sizeof(int2) * cmap.size(), b_c);
   char* d = (char*)g.d_tab;
-  g.d_scs = (RbScene*)d;
-  g.d_tstart = (int*)(d + b_scs);
-  g.d_rstart = (int*)(d + b_scs + b_i);
-  g.d_bpp = (int*)(d + b_scs + 2 * b_i);
-  g.d_brc = (int*)(d + b_scs + 3 * b_i);
-  g.d_bp = (int*)(d + b_scs + 4 * b_i);
-  g.d_cstart = (int64_t*)(d + b_scs + 5 * b_i);
+  g.d_scs = (RbScene*)(d + o_scs);
+  g.d_bpp = (int*)(d + o_bpp);
+  g.d_brc = (int*)(d + o_brc);
+  g.d_bp = (int*)(d + o_bp);
+  g.d_tmap = (int2*)(d + o_t);
+  g.d_rmap = (int2*)(d + o_r);
+  g.d_imap = (int2*)(d + o_m);
+  g.d_cmap = (int2*)(d + o_c);
   // behind the frames already queued on `st` (they read the old tables)
-  hipError_t e = hipMemcpyAsync(g.d_tab, h.data(), bytes, hipMemcpyHostToDevice, st);
+  hipError_t e = hipMemcpyAsync(g.d_tab, base, bytes, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
     err = std::string("red-black tables: ") + hipGetErrorString(e);
@@ -665,7 +782,7 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
 
 template <int MODE, int T, int NW, int R>
 static void launch_band(hipStream_t st, const RbGroup& g, int src, int dst, int x0, int coef, int nsw) {
-  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), dim3(g.tiles), dim3(64 * NW), 0, st, g.d_scs, g.d_tstart, g.n,
+  hipLaunchKernelGGL((rb::k_rb_band<MODE, T, NW, R>), dim3(g.tiles), dim3(64 * NW), 0, st, g.d_scs, g.d_tmap,
                      g.tiles, src, dst, x0, coef, nsw);
 }
 
@@ -688,8 +805,7 @@ static void launch_gs_mode(hipStream_t st, const RbGroup& g, int src, int dst, i
 }
 
 static void g_copy(hipStream_t st, const RbGroup& g, int dst, int src) {
-  hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)((g.cells + 255) / 256)), dim3(256), 0, st, g.d_scs, g.d_cstart, g.n,
-                     dst, src);
+  hipLaunchKernelGGL(rb::k_copy, dim3((unsigned)g.cblocks), dim3(256), 0, st, g.d_scs, g.d_cmap, dst, src);
 }
 
 // lin_solve in red-black order on every scene of the group (one iteration
@@ -725,35 +841,32 @@ static void g_bnd(hipStream_t st, const RbGroup& g, int mode0, int x0, int mode1
 // the iteration counts, so the sequence of launches is the same for all.
 static void g_frame(hipStream_t st, const RbGroup& g) {
   using rb::ROLE_NONE;
-  const dim3 rows((unsigned)((g.maxN + 255) / 256), (unsigned)g.rows);
-  const dim3 rows_adv((unsigned)((g.maxN + 256 * rb::ADV_K - 1) / (256 * rb::ADV_K)), (unsigned)g.rows);
+  const dim3 cells((unsigned)g.iblocks);  // interior cell blocks
   if (g.any_src)
-    hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)((g.cells + 255) / 256)), dim3(256), 0, st, g.d_scs, g.d_cstart,
-                       g.n);
+    hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)g.cblocks), dim3(256), 0, st, g.d_scs, g.d_cmap);
   g_lin_solve(st, g, M_ROW, F_U0, F_U, 0, g.diff_iters);  // fluid.rs:438
   g_lin_solve(st, g, M_COL, F_V0, F_V, 0, g.diff_iters);  // fluid.rs:448
   // project(U0, V0, p=U, div=V), fluid.rs:459
-  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0, F_V, F_U);
+  hipLaunchKernelGGL(rb::k_div, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0, F_V, F_U);
   g_bnd(st, g, M_PASSIVE, F_V, M_PASSIVE, F_U);
   g_lin_solve(st, g, M_PASSIVE, F_U, F_V, 2, g.press_iters);
-  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0, F_U);
+  hipLaunchKernelGGL(rb::k_grad, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0, F_U);
   g_bnd(st, g, M_ROW, F_U0, M_COL, F_V0);
   // advect U from U0 and V from V0 with (U0, V0), fluid.rs:469-489
-  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U0, F_V0);
-  hipLaunchKernelGGL(rb::k_adv, rows_adv, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_U0, F_V, F_V0, F_U0,
-                     F_V0);
+  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rmap, F_U0, F_V0);
+  hipLaunchKernelGGL(rb::k_adv, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_U0, F_V, F_V0, F_U0, F_V0);
   g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // project(U, V, p=U0, div=V0), fluid.rs:491
-  hipLaunchKernelGGL(rb::k_div, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V, F_V0, F_U0);
+  hipLaunchKernelGGL(rb::k_div, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_V, F_V0, F_U0);
   g_bnd(st, g, M_PASSIVE, F_V0, M_PASSIVE, F_U0);
   g_lin_solve(st, g, M_PASSIVE, F_U0, F_V0, 2, g.press_iters);
-  hipLaunchKernelGGL(rb::k_grad, rows, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V, F_U0);
+  hipLaunchKernelGGL(rb::k_grad, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_V, F_U0);
   g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // density, fluid.rs:501-523
   g_lin_solve(st, g, M_PASSIVE, F_SC, F_D, 1, g.diff_iters);
-  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_U, F_V);
-  hipLaunchKernelGGL(rb::k_adv, rows_adv, dim3(256), 0, st, g.d_scs, g.d_rstart, g.n, F_D, F_SC, ROLE_NONE,
-                     ROLE_NONE, F_U, F_V);
+  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rmap, F_U, F_V);
+  hipLaunchKernelGGL(rb::k_adv, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_D, F_SC, ROLE_NONE, ROLE_NONE,
+                     F_U, F_V);
   g_bnd(st, g, M_PASSIVE, F_D, M_PASSIVE, ROLE_NONE);
   g_copy(st, g, F_SC, F_D);
 }
