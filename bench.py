@@ -138,26 +138,37 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2):
            "sweeps_per_launch": tsw, "roofline_alg_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
                HBM_PEAK_GBS * 1e9)}
     if check_frames > 0:
-        errs = {}
+        errs, ref_errs = {}, {}
         for N in sorted({sc["size"] - 2 for sc in scs}):
             sc = [x for x in scs if x["size"] - 2 == N][0]
             g = mk(sc)
             g.step(check_frames)
             g.sync()
-            st = oracle.State.new(sc["size"]).astype(np.float64)
-            for r in sc["rects"]:
-                st.fill_rect(*r)
-            st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], check_frames, order=oracle.RB,
-                   src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
-            e = {}
+            def oracle_run(dtype, order):
+                st = oracle.State.new(sc["size"]).astype(dtype)
+                for r in sc["rects"]:
+                    st.fill_rect(*r)
+                st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], check_frames, order=order,
+                       src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+                return st
+
+            def rel(a, b):
+                a = a.astype(np.float64)
+                return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+            st = oracle_run(np.float64, oracle.RB)
+            # the reference's own f32 step (lexicographic order) against its f64 version, same scene
+            lex32, lex64 = oracle_run(np.float32, oracle.LEX), oracle_run(np.float64, oracle.LEX)
+            e, eref = {}, {}
             for name, fid in (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY)):
-                got = g.download(fid).astype(np.float64)
-                want = st.f[name]
-                e[name] = float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))
+                e[name] = rel(g.download(fid), st.f[name])
+                eref[name] = rel(lex32.f[name], lex64.f[name])
             g.close()
             errs[f"N={N}"] = e
+            ref_errs[f"N={N}"] = eref
         res["vs_fp64"] = {"frames": check_frames, "rel_norm": errs,
-                          "worst_rel_norm": max(max(v.values()) for v in errs.values())}
+                          "worst_rel_norm": max(max(v.values()) for v in errs.values()),
+                          "reference_f32_vs_fp64_rel_norm": ref_errs}
     return res
 
 

@@ -663,7 +663,7 @@ struct RBWork {
 struct BandCfg {
   int nw, r;
 };
-static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}};
+static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}, {8, 18}};
 
 static int band_tiles(int S, int tile, int T) {
   const BandCfg b = kBands[tile];
@@ -801,6 +801,7 @@ template <int MODE>
 static void launch_gs_mode(hipStream_t st, const RbGroup& g, int src, int dst, int x0, int coef, int nsw) {
   if (g.tile == 1) launch_band_t<MODE, 8, 32>(st, g, src, dst, x0, coef, nsw);
   else if (g.tile == 2) launch_band_t<MODE, 16, 16>(st, g, src, dst, x0, coef, nsw);
+  else if (g.tile == 3) launch_band_t<MODE, 8, 18>(st, g, src, dst, x0, coef, nsw);
   else launch_band_t<MODE, 8, 16>(st, g, src, dst, x0, coef, nsw);
 }
 
@@ -962,7 +963,7 @@ void rb_destroy(void* p) {
 }
 
 static bool tiling_ok(int tile, int tsweeps) {
-  return tile >= 0 && tile <= 2 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
+  return tile >= 0 && tile <= 3 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
 }
 
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {

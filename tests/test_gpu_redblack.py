@@ -20,7 +20,7 @@ def _case(label):
     print(f"[case] {label}", flush=True)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
 @pytest.mark.parametrize("tsw", [1, 2, 4, 5, 6, 8])
 def test_redblack_bit_exact_all_tilings(tile, tsw):
     N = 150
