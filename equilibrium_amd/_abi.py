@@ -432,7 +432,10 @@ class Server:
         return out
 
     def release(self, ticket):
+        """Forgets a ticket (cancelling it if queued or resident); borrowed
+        arrays stay referenced until close()."""
         check(lib().eqf_server_release(self.h, ticket))
+        self._keep.pop(ticket, None)
 
     def wait_idle(self, timeout_ms=-1.0):
         check(lib().eqf_server_wait_idle(self.h, timeout_ms))

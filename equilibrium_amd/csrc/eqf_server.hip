@@ -45,6 +45,7 @@ struct Req {
   bool keep_vel = false;
   int state = EQF_REQ_QUEUED;
   bool cancel = false;
+  bool forget = false;  // released while queued or resident: dropped from the ticket map once retired
   int64_t frames_done = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
   eqf_scene* scene = nullptr;
@@ -165,7 +166,7 @@ static void finish(eqf_server* sv, Req* r) {
 static void server_loop(eqf_server* sv) {
   (void)hipSetDevice(sv->ctx->device);
   std::vector<SceneDesc> descs;
-  std::vector<Req*> last_list;
+  std::vector<uint64_t> last_uids;  // scenes (by uid: never reused) whose descriptors d_descs holds
   for (;;) {
     std::vector<Req*> act;
     {
@@ -182,6 +183,7 @@ static void server_loop(eqf_server* sv) {
           sv->used_bytes -= r->bytes;
           if (r->state != EQF_REQ_FAILED) r->state = EQF_REQ_CANCELLED;
           it = sv->active.erase(it);
+          if (r->forget) sv->reqs.erase(r->ticket);
         } else {
           ++it;
         }
@@ -199,27 +201,36 @@ static void server_loop(eqf_server* sv) {
       std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
       int maxns = 1;
       for (Req* r : wf) maxns = std::max(maxns, r->scene->ns);
-      if (wf != last_list) {
+      std::vector<uint64_t> uids(wf.size());
+      for (size_t i = 0; i < wf.size(); ++i) uids[i] = wf[i]->scene->uid;
+      hipError_t e = hipSuccess;
+      if (uids != last_uids) {
         descs.resize(wf.size());
         for (size_t i = 0; i < wf.size(); ++i) {
           descs[i] = wf[i]->scene->desc;
           descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
         }
+        last_uids.clear();
         if ((int)wf.size() > sv->desc_cap) {
+          const int cap = std::max<int>((int)wf.size(), 2 * sv->desc_cap);
           (void)hipFree(sv->d_descs);
-          sv->desc_cap = std::max<int>((int)wf.size(), sv->desc_cap * 2);
-          (void)hipMalloc(&sv->d_descs, sizeof(SceneDesc) * sv->desc_cap);
+          sv->d_descs = nullptr;
+          sv->desc_cap = 0;
+          e = hipMalloc(&sv->d_descs, sizeof(SceneDesc) * cap);
+          if (e == hipSuccess) sv->desc_cap = cap;
+          else sv->d_descs = nullptr;
         }
-        (void)hipMemcpyAsync(sv->d_descs, descs.data(), sizeof(SceneDesc) * wf.size(), hipMemcpyHostToDevice,
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(sv->d_descs, descs.data(), sizeof(SceneDesc) * wf.size(), hipMemcpyHostToDevice,
                              sv->stream);
-        last_list = wf;
+        if (e == hipSuccess) last_uids = uids;
       }
       int lds = 0, waves = 1;
       for (Req* r : wf) {
         lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
         waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
       }
-      hipError_t e = eqf_internal_launch(sv->d_descs, (int)wf.size(), waves, lds, 1, sv->stream);
+      if (e == hipSuccess) e = eqf_internal_launch(sv->d_descs, (int)wf.size(), waves, lds, 1, sv->stream);
       if (e != hipSuccess) {
         std::lock_guard<std::mutex> lk(sv->mu);
         sv->thread_error = hipGetErrorString(e);
@@ -281,6 +292,7 @@ static void server_loop(eqf_server* sv) {
         if (h->cancel) {
           sv->queue.pop_front();
           h->state = EQF_REQ_CANCELLED;
+          if (h->forget) sv->reqs.erase(h->ticket);
           continue;
         }
         const size_t resident = sv->active.size() + admitted.size();
@@ -340,6 +352,7 @@ static void server_loop(eqf_server* sv) {
         sv->used_bytes -= r->bytes;
         if (r->state != EQF_REQ_FAILED) r->state = r->cancel ? EQF_REQ_CANCELLED : EQF_REQ_DONE;
         r->t_done = sv->now_ms();
+        if (r->forget) sv->reqs.erase(r->ticket);
       }
       for (Req* r : admitted) {
         // Staging buffers are only needed again for the final read-back; free
@@ -348,8 +361,9 @@ static void server_loop(eqf_server* sv) {
         if (r->nframes <= 0) {  // nothing to run
           finish(sv, r);
           sv->used_bytes -= r->bytes;
-          r->state = EQF_REQ_DONE;
+          r->state = r->cancel ? EQF_REQ_CANCELLED : EQF_REQ_DONE;
           r->t_done = sv->now_ms();
+          if (r->forget) sv->reqs.erase(r->ticket);
           continue;
         }
         r->state = EQF_REQ_ACTIVE;
@@ -471,7 +485,8 @@ extern "C" int eqf_server_release(eqf_server* sv, uint64_t ticket) {
   if (it == sv->reqs.end()) return eqf_internal_error(EQF_ENOTFOUND, "unknown ticket");
   Req* r = it->second.get();
   if (r->state == EQF_REQ_QUEUED || r->state == EQF_REQ_ACTIVE) {
-    r->cancel = true;  // the server thread frees it at the next tick
+    r->cancel = true;  // the server thread retires it and forgets the ticket
+    r->forget = true;
     sv->cv.notify_all();
     return EQF_OK;
   }

@@ -89,11 +89,19 @@ def test_server_rejects_oversized_and_cancels():
     t_long = srv.submit(long_sc)
     ts = [srv.submit(sc) for sc in mix[:4]]
     time.sleep(0.2)
-    srv.release(t_long)
+    srv.release(t_long)  # cancels the resident request and forgets its ticket once retired
+    t_queued = srv.submit(long_sc)
+    srv.release(t_queued)
     srv.wait_idle(60000)
-    assert srv.poll(t_long).state == _abi.REQ_CANCELLED
+    for t in (t_long, t_queued):
+        with pytest.raises(_abi.EqfError) as ei:
+            srv.poll(t)
+        assert ei.value.code == _abi.EQF_ENOTFOUND
     for sc, t in zip(mix[:4], ts):
         assert_bit_equal(srv.result(t), _solo(sc)[0], f"cancel N={sc['size'] - 2}")
+        srv.release(t)
+        with pytest.raises(_abi.EqfError):
+            srv.poll(t)
     srv.close()
 
 
