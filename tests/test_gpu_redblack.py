@@ -35,7 +35,8 @@ def test_redblack_bit_exact_all_tilings(tile, tsw):
     assert_state_equal(download_all(s), st, f"rb tile={tile} T={tsw}")
 
 
-@pytest.mark.parametrize("N,iters,frames", [(8, 3, 2), (30, 5, 3), (64, 20, 2), (201, 9, 2)])
+@pytest.mark.parametrize("N,iters,frames", [(1, 2, 2), (2, 3, 2), (3, 3, 2), (8, 3, 2), (30, 5, 3), (31, 4, 2),
+                                           (33, 4, 2), (64, 20, 2), (201, 9, 2)])
 def test_redblack_bit_exact_sizes(N, iters, frames):
     _case(f"rb N={N}")
     sc = scenes.stam_scene(N, scenes.derive_seed(21, N), iters=iters, frames=frames, n_rects=3)
@@ -123,6 +124,26 @@ def test_redblack_batch_equals_solo_and_oracle():
                 assert np.array_equal(got[name].view(np.uint32), want[k][name].view(np.uint32)), \
                     (tile, tsw, sc["size"], name)
             h.close()
+
+
+def test_redblack_batch_of_many_small_scenes():
+    """48 scenes of sizes 1..200 in one red-black batch: work units of many
+    scenes share every launch (tiles, rows, cell blocks straddling nothing),
+    each scene equal to the red-black oracle bit for bit."""
+    _case("rb batch small")
+    rng = np.random.default_rng(24)
+    sizes = [1, 2, 3, 5, 16, 31, 32, 33] + [int(x) for x in rng.integers(4, 200, 40)]
+    mix = [scenes.stam_scene(N, scenes.derive_seed(25, k), iters=5, frames=2, n_rects=k % 3,
+                             viscosity=10 ** (-3 - k % 3), diffusion=10 ** (-5 + k % 2))
+           for k, N in enumerate(sizes)]
+    hs = [gpu_scene(sc, _abi.GS_REDBLACK) for sc in mix]
+    b = _abi.Batch(hs)
+    b.step(2)
+    b.sync()
+    b.close()
+    for sc, h in zip(mix, hs):
+        assert_state_equal(download_all(h), run_oracle(sc, order=oracle.RB), f"rb batch N={sc['size'] - 2}")
+        h.close()
 
 
 def test_redblack_batch_rejects_mixed_modes_and_counts():
