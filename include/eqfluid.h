@@ -221,7 +221,9 @@ int eqf_server_poll(eqf_server *sv, uint64_t ticket, eqf_request_status *st);
 /* Copies a finished request's field (EQF_DENSITY, or EQF_VX/EQF_VY when
  * keep_velocity was set) into `out` (size*size floats). */
 int eqf_server_result(eqf_server *sv, uint64_t ticket, int field, float *out);
-/* Forgets a ticket; cancels it if it is queued or resident. */
+/* Forgets a ticket: a finished request's record is dropped at once; a queued
+ * or resident one is cancelled and dropped when the server retires it (poll
+ * then reports EQF_ENOTFOUND). */
 int eqf_server_release(eqf_server *sv, uint64_t ticket);
 /* Waits until no request is queued or resident (timeout_ms < 0: forever). */
 int eqf_server_wait_idle(eqf_server *sv, double timeout_ms);
