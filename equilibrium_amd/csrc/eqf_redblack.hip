@@ -70,10 +70,10 @@ enum BandSel : uint32_t {
 // One scene of a red-black launch group: a single scene, or every scene of
 // a batch.  Each kernel below covers all scenes of its group in one launch.
 // A workgroup finds its scene and its place in it with one load from a map
-// of the launch's work units (band tile, interior row, block of CELL_BLOCK
-// interior cells or of CELL_BLOCK cells ring included -> {scene, index in
-// the scene}); the set_boundaries list kernel, whose threads straddle
-// scenes, searches a prefix table instead.
+// of the launch's work units (band tile, block of CELL_BLOCK interior cells
+// or of CELL_BLOCK cells ring included -> {scene, index in the scene}); the
+// set_boundaries list kernel, whose threads straddle scenes, searches a
+// prefix table instead.
 struct RbScene {
   SceneDesc d;  // the scene's descriptor (a copy): fields, sizes, coefficients, rule bytes, lists
   float* tmp;   // its ping-pong field
@@ -339,7 +339,8 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__
 
 // project, first half (fluid.rs:339-349).  imap: workgroup -> {scene, block
 // of CELL_BLOCK interior cells}; every load of a thread's cells is issued
-// before its stores.
+// before its stores.  Also resets every row's break column to N+1 for the
+// advect that follows this projection (the cell in column 1 does it).
 __global__ void __launch_bounds__(256) k_div(const RbScene* __restrict__ scs, const int2* __restrict__ imap,
                                              int u_role, int v_role, int div_role, int p_role) {
   const int2 im = imap[blockIdx.x];
@@ -368,6 +369,7 @@ __global__ void __launch_bounds__(256) k_div(const RbScene* __restrict__ scs, co
     const int q = cj[k] * S + ci[k];
     dv[q] = -0.5f * (ur[k] - ul[k] + vd[k] - vu[k]) / sf;
     p[q] = 0.0f;
+    if (ci[k] == 1) sc.trig[cj[k]] = N + 1;
   }
 }
 
@@ -433,46 +435,40 @@ __device__ __forceinline__ float gather(const float* __restrict__ d0, const Trac
 }
 
 // advect, pass A: first column of each row whose back-trace reaches the far
-// edge (the `break` of fluid.rs:420-423); N+1 when none.  One workgroup per
-// row of any scene; each thread tests 8 columns per round with all 16 loads
-// in flight.
-__global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ scs, const int2* __restrict__ rmap,
+// edge (the `break` of fluid.rs:420-423), as the minimum of the triggering
+// columns (k_div set every row to N+1 "none" before).  imap as for k_div.  A
+// wave's lanes hold consecutive cells, so only the leftmost triggering cell
+// of each row within a wave updates the row's minimum.
+__global__ void __launch_bounds__(256) k_adv_trig(const RbScene* __restrict__ scs, const int2* __restrict__ imap,
                                                   int u_role, int v_role) {
-  const int2 rm = rmap[blockIdx.x];
-  const RbScene& sc = scs[rm.x];
+  const int2 im = imap[blockIdx.x];
+  const RbScene& sc = scs[im.x];
   const SceneDesc& d = sc.d;
   const int S = d.S, N = d.N;
-  const int j = rm.y;
+  int ci[CELL_K], cj[CELL_K];
+  bool ok[CELL_K];
+  interior_cells(im.y, N, ci, cj, ok);
   const float* __restrict__ u = role_ptr(sc, u_role);
   const float* __restrict__ v = role_ptr(sc, v_role);
-  const int64_t row = (int64_t)j * S;
-  constexpr int K = 8;
-  int t = N + 1;
-  for (int base = 1; base <= N; base += 256 * K) {
-    float uu[K], vv[K];
+  float uu[CELL_K], vv[CELL_K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int i = min(base + k * 256 + (int)threadIdx.x, N);  // in-row address; tested below
-      uu[k] = u[row + i];
-      vv[k] = v[row + i];
-    }
+  for (int k = 0; k < CELL_K; ++k) {
+    const int q = cj[k] * S + ci[k];
+    uu[k] = u[q];
+    vv[k] = v[q];
+  }
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int i = base + k * 256 + threadIdx.x;
-      if (i <= N && back_trace(d, i, j, uu[k], vv[k]).trig) t = min(t, i);
+  for (int k = 0; k < CELL_K; ++k) {
+    const bool t = ok[k] && back_trace(d, ci[k], cj[k], uu[k], vv[k]).trig;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(t);
+    if (t) {
+      // lanes max(0, lane - (i-1)) .. lane-1 hold this row's cells left of this one
+      const int first = max(0, lane - (ci[k] - 1));
+      const uint64_t left = ((1ull << lane) - 1) & ~((1ull << first) - 1);
+      if ((m & left) == 0) atomicMin(&sc.trig[cj[k]], ci[k]);
     }
-    // later rounds hold only larger columns: the block stops together after
-    // the first round in which any column triggered
-    if (__syncthreads_or(t <= N)) break;
   }
-  __shared__ int red[256];
-  red[threadIdx.x] = t;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = min(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) sc.trig[j] = red[0];
 }
 
 // advect, pass B (fluid.rs:398-430) for one or two fields sharing u, v
@@ -639,10 +635,10 @@ struct RbGroup {
   size_t tab_bytes = 0;
   std::vector<char> h_tab;  // host image of the tables
   RbScene* d_scs = nullptr;
-  // work unit -> {scene, index in it}: band tiles, interior rows, interior cell blocks, cell blocks
-  int2 *d_tmap = nullptr, *d_rmap = nullptr, *d_imap = nullptr, *d_cmap = nullptr;
+  // work unit -> {scene, index in it}: band tiles, interior cell blocks, cell blocks
+  int2 *d_tmap = nullptr, *d_imap = nullptr, *d_cmap = nullptr;
   int *d_bpp = nullptr, *d_brc = nullptr, *d_bp = nullptr;  // prefixes of list entries
-  int n = 0, rows = 0, maxN = 0, tiles = 0, iblocks = 0, cblocks = 0, bpp = 0, brc = 0, bp = 0;
+  int n = 0, maxN = 0, tiles = 0, iblocks = 0, cblocks = 0, bpp = 0, brc = 0, bp = 0;
   bool any_src = false;
   int diff_iters = 0, press_iters = 0;
   int tile = 0, tsweeps = 4;  // band shape index (kBands) and sweeps per launch
@@ -692,7 +688,7 @@ static uint64_t group_key(const RbGroup& g) {
 static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
   const int n = (int)g.scenes.size();
   std::vector<RbScene> scs(n);
-  std::vector<int2> tmap, rmap, imap, cmap;
+  std::vector<int2> tmap, imap, cmap;
   std::vector<int> bpp(n + 1, 0), brc(n + 1, 0), bp(n + 1, 0);
   g.any_src = false;
   g.maxN = 0;
@@ -702,7 +698,6 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
     scs[i] = RbScene{s->desc, w->tmp, w->trig};
     const int nt = band_tiles(s->S, g.tile, g.tsweeps);
     for (int t = 0; t < nt; ++t) tmap.push_back(make_int2(i, t));
-    for (int j = 1; j <= s->N; ++j) rmap.push_back(make_int2(i, j));
     const int64_t ni = ((int64_t)s->N * s->N + rb::CELL_BLOCK - 1) / rb::CELL_BLOCK;
     for (int64_t k = 0; k < ni; ++k) imap.push_back(make_int2(i, (int)k));
     const int64_t nb = ((int64_t)s->S * s->S + rb::CELL_BLOCK - 1) / rb::CELL_BLOCK;
@@ -713,13 +708,12 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
     g.any_src = g.any_src || s->desc.has_src;
     g.maxN = std::max(g.maxN, s->N);
   }
-  if (tmap.size() > INT32_MAX / 2 || rmap.size() > INT32_MAX / 2 || cmap.size() > INT32_MAX / 2) {  // imap < cmap
+  if (tmap.size() > INT32_MAX / 2 || cmap.size() > INT32_MAX / 2) {  // imap < cmap
     err = "red-black launch group too large for one launch";
     return EQF_EINVAL;
   }
   g.n = n;
   g.tiles = (int)tmap.size();
-  g.rows = (int)rmap.size();
   g.iblocks = (int)imap.size();
   g.cblocks = (int)cmap.size();
   g.bpp = bpp[n];
@@ -727,9 +721,9 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
   g.bp = bp[n];
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t b_scs = up(sizeof(RbScene) * n), b_i = up(sizeof(int) * (n + 1));
-  const size_t b_t = up(sizeof(int2) * tmap.size()), b_r = up(sizeof(int2) * rmap.size());
+  const size_t b_t = up(sizeof(int2) * tmap.size());
   const size_t b_m = up(sizeof(int2) * imap.size()), b_c = up(sizeof(int2) * cmap.size());
-  const size_t bytes = b_scs + 3 * b_i + b_t + b_r + b_m + b_c;
+  const size_t bytes = b_scs + 3 * b_i + b_t + b_m + b_c;
   if (bytes > g.tab_bytes) {  // with headroom: a server's group changes size often
     (void)hipFree(g.d_tab);
     g.d_tab = nullptr;
@@ -756,7 +750,6 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
   const size_t o_brc = put(brc.data(), sizeof(int) * (n + 1), b_i);
   const size_t o_bp = put(bp.data(), sizeof(int) * (n + 1), b_i);
   const size_t o_t = put(tmap.data(), sizeof(int2) * tmap.size(), b_t);
-  const size_t o_r = put(rmap.data(), sizeof(int2) * rmap.size(), b_r);
   const size_t o_m = put(imap.data(), sizeof(int2) * imap.size(), b_m);
   const size_t o_c = put(cmap.data(), sizeof(int2) * cmap.size(), b_c);
   char* d = (char*)g.d_tab;
@@ -765,7 +758,6 @@ static int group_tables(RbGroup& g, hipStream_t st, std::string& err) {
   g.d_brc = (int*)(d + o_brc);
   g.d_bp = (int*)(d + o_bp);
   g.d_tmap = (int2*)(d + o_t);
-  g.d_rmap = (int2*)(d + o_r);
   g.d_imap = (int2*)(d + o_m);
   g.d_cmap = (int2*)(d + o_c);
   // behind the frames already queued on `st` (they read the old tables)
@@ -854,7 +846,7 @@ static void g_frame(hipStream_t st, const RbGroup& g) {
   hipLaunchKernelGGL(rb::k_grad, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0, F_U);
   g_bnd(st, g, M_ROW, F_U0, M_COL, F_V0);
   // advect U from U0 and V from V0 with (U0, V0), fluid.rs:469-489
-  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rmap, F_U0, F_V0);
+  hipLaunchKernelGGL(rb::k_adv_trig, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0);
   hipLaunchKernelGGL(rb::k_adv, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_U0, F_V, F_V0, F_U0, F_V0);
   g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // project(U, V, p=U0, div=V0), fluid.rs:491
@@ -865,7 +857,7 @@ static void g_frame(hipStream_t st, const RbGroup& g) {
   g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
   // density, fluid.rs:501-523
   g_lin_solve(st, g, M_PASSIVE, F_SC, F_D, 1, g.diff_iters);
-  hipLaunchKernelGGL(rb::k_adv_trig, dim3((unsigned)g.rows), dim3(256), 0, st, g.d_scs, g.d_rmap, F_U, F_V);
+  hipLaunchKernelGGL(rb::k_adv_trig, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_V);
   hipLaunchKernelGGL(rb::k_adv, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_D, F_SC, ROLE_NONE, ROLE_NONE,
                      F_U, F_V);
   g_bnd(st, g, M_PASSIVE, F_D, M_PASSIVE, ROLE_NONE);
