@@ -101,7 +101,7 @@ def bench_batch(scs, steps, warmup, tuning=None):
     return wall, ev_ms, {"max_waves": w, "order": "largest first" if o == 0 else "given", "trial_ms": timings}
 
 
-def bench_batch_redblack(scs, steps, warmup, check_frames=2):
+def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
     """The same mix in red-black mode: every frame kernel covers all 512
     scenes (band passes over all scenes' tiles with register temporal
     blocking), band shape and sweeps per launch autotuned; accuracy vs the f64
@@ -117,8 +117,12 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2):
     hs = [mk(sc) for sc in scs]
     for h in hs:
         h.sync()
-    tile, tsw, timings = tune.tune_batch_band(hs)
-    log(f"red-black batch: autotuned band {tune.TILES[tile]} x {tsw} sweeps/launch: {timings}")
+    if band:
+        tile, tsw = band
+        log(f"red-black batch: band {tune.TILES[tile]} x {tsw} sweeps/launch (given)")
+    else:
+        tile, tsw, timings = tune.tune_batch_band(hs)
+        log(f"red-black batch: autotuned band {tune.TILES[tile]} x {tsw} sweeps/launch: {timings}")
     b = _abi.Batch(hs)
     b.set_band(tile, tsw)
     if warmup:
@@ -422,6 +426,8 @@ def main():
                     help="max_waves:order for the batch instead of autotuning (e.g. 16:0)")
     ap.add_argument("--rb-tiling", type=str, default="",
                     help="band:sweeps for the N=4096 red-black grid instead of autotuning (e.g. 0:5)")
+    ap.add_argument("--rb-batch-band", type=str, default="",
+                    help="band:sweeps for the red-black batched mix instead of autotuning (e.g. 3:6)")
     ap.add_argument("--snap-frames", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-rates", type=str, default="50,500,2000")
@@ -500,7 +506,8 @@ def main():
 
     if "rbmix" in legs:
         # the batched mix in red-black mode (<= 1e-5 vs FP64 instead of bit-exact)
-        out["batched_mix_redblack"] = bench_batch_redblack(mix, args.steps, args.warmup)
+        band = tuple(int(x) for x in args.rb_batch_band.split(":")) if args.rb_batch_band else None
+        out["batched_mix_redblack"] = bench_batch_redblack(mix, args.steps, args.warmup, args.rb_check_frames, band)
         log(f"red-black batch: {out['batched_mix_redblack']}")
 
     if "single" in legs:

@@ -48,6 +48,8 @@ enum CodeBits : uint32_t {
                     // (c, j) is a wall and (c, j-2) is not: the wave that
                     // computes (c, j) writes the AdjustColumn value of (c, j-1)
   C_COLF = 64u,     // column c holds at least one NoWall cell (Passive ring copy)
+  C_RING = 128u,    // ring cell (no corner) that set_boundaries(Passive) rewrites:
+                    // its row / column holds a NoWall cell (fluid.rs:179-187)
 };
 
 // Boundary selectors of a CYCLIC scene: two bits per interior cell and mode

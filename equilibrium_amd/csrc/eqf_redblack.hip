@@ -25,6 +25,8 @@
 
 #include "eqf_internal.h"
 
+#define GAS __attribute__((address_space(1)))
+
 
 
 namespace eqf {
@@ -149,7 +151,7 @@ __device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, fl
 // buffer and the tile is written to the other (ping-pong).  Corners are set
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
-__global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((NW == 8 && R <= 18) ? 4 : 1))) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
                                                      int nt, int src_role, int dst_role, int x0_role, int coef,
                                                      int nsweeps) {
   constexpr int H = 3 * T, W = 128, RH = NW * R;
@@ -176,61 +178,72 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__
   const int S = d.S, N = d.N;
   const int tiles_x = (S + TX - 1) / TX;
   const int bx = tl % tiles_x, by = tl / tiles_x;
-  const float* __restrict__ src = role_ptr(sc, src_role);
-  float* __restrict__ dst = role_ptr(sc, dst_role);
-  const float* __restrict__ x0g = role_ptr(sc, x0_role);
-  const uint8_t* __restrict__ code = d.code;
-  const uint8_t* __restrict__ rowf = d.rowf;
+  // Global-memory views: a row's base address is wave-uniform (scalar
+  // registers), a lane's column a 32-bit offset.
+  GAS const float* __restrict__ src = (GAS const float*)role_ptr(sc, src_role);
+  GAS float* __restrict__ dst = (GAS float*)role_ptr(sc, dst_role);
+  GAS const float* __restrict__ x0g = (GAS const float*)role_ptr(sc, x0_role);
+  GAS const uint8_t* __restrict__ code = (GAS const uint8_t*)d.code;
   float a, cr;
   coefs(d, coef, a, cr);
   const int gx0 = bx * TX - H;  // region origin; gx0 + gy0 is even (TX, TY, R even)
   const int gyb = by * TY - H;
   const int gy0 = gyb + w * R;          // this wave's band
   const int gxa = gx0 + 2 * lane;
+  // this lane's two columns, clamped into the grid (cells outside it are
+  // replaced by 0 after the loads), and whether they are in the grid / interior
+  uint32_t col[2];
+  bool cin[2], cint[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int gx = gxa + b;
+    col[b] = (uint32_t)min(max(gx, 0), S - 1);
+    cin[b] = gx >= 0 && gx < S;
+    cint[b] = gx >= 1 && gx <= N;
+  }
   float x[R][2], x0[R][2];
   uint32_t cd[R];
-  // Loads first, all of them, from clamped in-grid addresses (cells outside
-  // the grid are replaced by 0 afterwards), so that one wait covers every
-  // load of the region.
+  // Loads first, all of them, so that one wait covers every load of the region.
   uint32_t craw[R][2];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    const int64_t row = (int64_t)min(max(gy0 + i, 0), S - 1) * S;
+    const int64_t row = (int64_t)min(max(gy0 + i, 0), S - 1) * S;  // wave-uniform
+    // scalar row base + 32-bit byte offset: one VGPR per column, shared by all rows
+    GAS const char* sp = (GAS const char*)(src + row);
+    GAS const char* xp = (GAS const char*)(x0g + row);
+    GAS const uint8_t* cp = code + row;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const int64_t g = row + min(max(gxa + b, 0), S - 1);
-      x[i][b] = src[g];
-      x0[i][b] = x0g[g];
-      if constexpr (MODE != M_PASSIVE) craw[i][b] = code[g];
+      x[i][b] = *(GAS const float*)(sp + 4u * col[b]);
+      x0[i][b] = *(GAS const float*)(xp + 4u * col[b]);
+      craw[i][b] = cp[col[b]];
     }
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int gy = gy0 + i;
+    const bool rin = gy >= 0 && gy < S, rint = gy >= 1 && gy <= N;  // wave-uniform
     uint32_t c2 = 0;
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int gx = gxa + b;
-      const bool in = gx >= 0 && gx < S && gy >= 0 && gy < S;
-      if (!in) {
-        x[i][b] = 0.0f;
-        x0[i][b] = 0.0f;
-      }
-      uint32_t sel = 0;
-      if (gx >= 1 && gx <= N && gy >= 1 && gy <= N) {
-        sel = BS_UPD;
-        if constexpr (MODE != M_PASSIVE) {
-          const uint32_t c = craw[i][b];
-          if (MODE == M_ROW && (c & C_FLUID)) sel |= (c & C_RWALL) ? BS_NEG_RIGHT : (c & C_LWALL) ? BS_NEG_LEFT : 0;
-          if (MODE == M_COL && (c & C_FLUID)) sel |= (c & C_UWALL) ? BS_NEG_UP : (c & C_DWALL) ? BS_NEG_DOWN : 0;
-        }
-      } else if (MODE == M_PASSIVE && in) {
-        // a ring cell of column i / row j is rewritten iff that column / row
-        // has a NoWall cell (ring cells only: a rare branch)
-        if (gy == 0 && gx >= 1 && gx <= N) sel = (code[S + gx] & C_COLF) ? BS_DOWN : 0;
-        else if (gy == S - 1 && gx >= 1 && gx <= N) sel = (code[(int64_t)(S - 2) * S + gx] & C_COLF) ? BS_UP : 0;
-        else if (gx == 0 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_RIGHT : 0;
-        else if (gx == S - 1 && gy >= 1 && gy <= N) sel = rowf[gy] ? BS_LEFT : 0;
+    for (int b = 0; b < 2; ++b) {  // selects only: no divergent branch in the prologue
+      const bool inside = rin && cin[b], interior = rint && cint[b];
+      x[i][b] = inside ? x[i][b] : 0.0f;
+      x0[i][b] = inside ? x0[i][b] : 0.0f;
+      const uint32_t c = craw[i][b];
+      const bool fl = interior && (c & C_FLUID);
+      uint32_t sel = interior ? (uint32_t)BS_UPD : 0u;
+      if constexpr (MODE == M_ROW)
+        sel |= fl ? ((c & C_RWALL) ? (uint32_t)BS_NEG_RIGHT : (c & C_LWALL) ? (uint32_t)BS_NEG_LEFT : 0u) : 0u;
+      if constexpr (MODE == M_COL)
+        sel |= fl ? ((c & C_UWALL) ? (uint32_t)BS_NEG_UP : (c & C_DWALL) ? (uint32_t)BS_NEG_DOWN : 0u) : 0u;
+      if constexpr (MODE == M_PASSIVE) {
+        // a ring cell whose row / column holds a NoWall cell copies its
+        // interior neighbour (C_RING is set on such ring cells only)
+        const uint32_t dir = gy == 0       ? (uint32_t)BS_DOWN
+                             : gy == S - 1 ? (uint32_t)BS_UP
+                             : gxa + b == 0 ? (uint32_t)BS_RIGHT
+                                            : (uint32_t)BS_LEFT;
+        sel |= (inside && !interior && (c & C_RING)) ? dir : 0u;
       }
       c2 |= sel << (8 * b);
     }
@@ -325,15 +338,22 @@ __global__ void __launch_bounds__(64 * NW) k_rb_band(const RbScene* __restrict__
     }
   }
 
+  // (columns recomputed here rather than kept live through the sweeps)
+  const int gxl = bx * TX - H + 2 * (int)(threadIdx.x & 63);
+  bool cout[2];  // this lane's columns lie in the output tile
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int gx = gxl + b;
+    cout[b] = gx >= gx0 + H && gx < gx0 + H + TX && gx < S;
+  }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int gy = gy0 + i;
-    if (gy < gyb + H || gy >= gyb + H + TY || gy >= S) continue;
+    if (gy < gyb + H || gy >= gyb + H + TY || gy >= S) continue;  // wave-uniform
+    GAS char* dp = (GAS char*)(dst + (int64_t)gy * S);
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int gx = gxa + b;
-      if (gx >= gx0 + H && gx < gx0 + H + TX && gx < S) dst[(int64_t)gy * S + gx] = x[i][b];
-    }
+    for (int b = 0; b < 2; ++b)
+      if (cout[b]) *(GAS float*)(dp + 4u * (uint32_t)(gxl + b)) = x[i][b];
   }
 }
 

@@ -148,6 +148,7 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (on) {
       const int k = atomicAdd(&cnt[2], 1);
       if (fill) lpas[k] = make_int2((int)off_of(g, x, y), (int)off_of(g, sx, sy));
+      if (fill && g.layout == L_ROWMAJOR) code[e] = (uint8_t)C_RING;  // row-major: e == y*S + x
     }
   }
 }
