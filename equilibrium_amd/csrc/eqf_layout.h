@@ -36,8 +36,8 @@ enum Field { F_D = 0, F_SC = 1, F_U = 2, F_V = 3, F_U0 = 4, F_V0 = 5, NFIELDS = 
 enum Mode { M_ROW = 0, M_COL = 1, M_PASSIVE = 2 };  // Orientation, fluid.rs:22-29
 enum Layout { L_CYCLIC = 0, L_ROWMAJOR = 1 };
 
-// Per-cell boundary code of a ROWMAJOR scene (one byte per interior cell,
-// laid out like a field).
+// Boundary facts of an interior cell, computed while the boundary codes are
+// built (k_codes).
 enum CodeBits : uint32_t {
   C_FLUID = 1u,     // cell is ContainerWall::NoWall
   C_LWALL = 2u,     // (c-1, j) is a wall
@@ -48,8 +48,21 @@ enum CodeBits : uint32_t {
                     // (c, j) is a wall and (c, j-2) is not: the wave that
                     // computes (c, j) writes the AdjustColumn value of (c, j-1)
   C_COLF = 64u,     // column c holds at least one NoWall cell (Passive ring copy)
-  C_RING = 128u,    // ring cell (no corner) that set_boundaries(Passive) rewrites:
-                    // its row / column holds a NoWall cell (fluid.rs:179-187)
+};
+
+// Per-cell rule byte of a ROWMAJOR scene (one byte per cell, ring included,
+// laid out like a field), read by the red-black sweeps:
+//   bit 0     interior cell (updated by the sweeps)
+//   bits 1-2  AdjustRow: 0 keep, 1 -(left), 2 -(right)  (right wins, fluid.rs:151-164)
+//   bits 3-4  AdjustColumn: 0 keep, 1 -(up), 2 -(down)  (up wins, fluid.rs:165-177)
+//   bits 5-7  Passive, ring cells whose row / column holds a NoWall cell
+//             (fluid.rs:179-187): 0 none, 1 copy the left, 2 the right, 3 the
+//             upper, 4 the lower neighbour
+enum RuleBits : uint32_t {
+  R_INTERIOR = 1u,
+  R_ROW_SHIFT = 1, R_COL_SHIFT = 3, R_PAS_SHIFT = 5,
+  R_NEG_FIRST = 1u, R_NEG_SECOND = 2u,  // left / up, right / down
+  R_COPY_LEFT = 1u, R_COPY_RIGHT = 2u, R_COPY_UP = 3u, R_COPY_DOWN = 4u,
 };
 
 // Boundary selectors of a CYCLIC scene: two bits per interior cell and mode

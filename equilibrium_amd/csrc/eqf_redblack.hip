@@ -191,15 +191,14 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
   const int gy0 = gyb + w * R;          // this wave's band
   const int gxa = gx0 + 2 * lane;
   // this lane's two columns, clamped into the grid (cells outside it are
-  // replaced by 0 after the loads), and whether they are in the grid / interior
+  // replaced by 0 after the loads), and whether they are in the grid
   uint32_t col[2];
-  bool cin[2], cint[2];
+  bool cin[2];
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     const int gx = gxa + b;
     col[b] = (uint32_t)min(max(gx, 0), S - 1);
     cin[b] = gx >= 0 && gx < S;
-    cint[b] = gx >= 1 && gx <= N;
   }
   float x[R][2], x0[R][2];
   uint32_t cd[R];
@@ -219,33 +218,30 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
       craw[i][b] = cp[col[b]];
     }
   }
+  // The rule byte (eqf_layout.h RuleBits) of each cell, as this mode's
+  // selector; cells outside the grid (clamped loads) get none.
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int gy = gy0 + i;
-    const bool rin = gy >= 0 && gy < S, rint = gy >= 1 && gy <= N;  // wave-uniform
+    const bool rin = gy >= 0 && gy < S;  // wave-uniform
     uint32_t c2 = 0;
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {  // selects only: no divergent branch in the prologue
-      const bool inside = rin && cin[b], interior = rint && cint[b];
+    for (int b = 0; b < 2; ++b) {
+      const bool inside = rin && cin[b];
       x[i][b] = inside ? x[i][b] : 0.0f;
       x0[i][b] = inside ? x0[i][b] : 0.0f;
       const uint32_t c = craw[i][b];
-      const bool fl = interior && (c & C_FLUID);
-      uint32_t sel = interior ? (uint32_t)BS_UPD : 0u;
-      if constexpr (MODE == M_ROW)
-        sel |= fl ? ((c & C_RWALL) ? (uint32_t)BS_NEG_RIGHT : (c & C_LWALL) ? (uint32_t)BS_NEG_LEFT : 0u) : 0u;
-      if constexpr (MODE == M_COL)
-        sel |= fl ? ((c & C_UWALL) ? (uint32_t)BS_NEG_UP : (c & C_DWALL) ? (uint32_t)BS_NEG_DOWN : 0u) : 0u;
-      if constexpr (MODE == M_PASSIVE) {
-        // a ring cell whose row / column holds a NoWall cell copies its
-        // interior neighbour (C_RING is set on such ring cells only)
-        const uint32_t dir = gy == 0       ? (uint32_t)BS_DOWN
-                             : gy == S - 1 ? (uint32_t)BS_UP
-                             : gxa + b == 0 ? (uint32_t)BS_RIGHT
-                                            : (uint32_t)BS_LEFT;
-        sel |= (inside && !interior && (c & C_RING)) ? dir : 0u;
+      uint32_t sel = (c & R_INTERIOR) ? (uint32_t)BS_UPD : 0u;
+      if constexpr (MODE == M_ROW) sel |= (c >> R_ROW_SHIFT) & 3u;  // BS_NEG_LEFT / BS_NEG_RIGHT
+      if constexpr (MODE == M_COL) {
+        const uint32_t r = (c >> R_COL_SHIFT) & 3u;
+        sel |= r ? r + (BS_NEG_UP - R_NEG_FIRST) : 0u;
       }
-      c2 |= sel << (8 * b);
+      if constexpr (MODE == M_PASSIVE) {
+        const uint32_t r = (c >> R_PAS_SHIFT) & 7u;
+        sel |= r ? r + (BS_LEFT - R_COPY_LEFT) : 0u;
+      }
+      c2 |= (inside ? sel : 0u) << (8 * b);
     }
     cd[i] = c2;
   }

@@ -104,7 +104,14 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (wall(x, y + 1)) c |= C_DWALL;
     if (wall(x, y - 1)) c |= C_UWALL;
     if (colf[x]) c |= C_COLF;
-    if (fill && g.layout == L_ROWMAJOR) code[off_of(g, x, y)] = (uint8_t)c;
+    if (fill && g.layout == L_ROWMAJOR) {
+      uint32_t r = R_INTERIOR;
+      if (fl) {
+        r |= ((c & C_RWALL) ? R_NEG_SECOND : (c & C_LWALL) ? R_NEG_FIRST : 0u) << R_ROW_SHIFT;
+        r |= ((c & C_UWALL) ? R_NEG_FIRST : (c & C_DWALL) ? R_NEG_SECOND : 0u) << R_COL_SHIFT;
+      }
+      code[off_of(g, x, y)] = (uint8_t)r;
+    }
     if (fill && g.layout == L_CYCLIC) {
       const int64_t o = cyc_code_word(x, y, g.P), mstride = cyc_code_bytes(g.P, g.ns) / 4;
       const int sh = cyc_code_shift(x, y, g.P);
@@ -148,7 +155,10 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (on) {
       const int k = atomicAdd(&cnt[2], 1);
       if (fill) lpas[k] = make_int2((int)off_of(g, x, y), (int)off_of(g, sx, sy));
-      if (fill && g.layout == L_ROWMAJOR) code[e] = (uint8_t)C_RING;  // row-major: e == y*S + x
+      if (fill && g.layout == L_ROWMAJOR) {  // row-major: e == y*S + x
+        const uint32_t r = y == 0 ? R_COPY_DOWN : y == S - 1 ? R_COPY_UP : x == 0 ? R_COPY_RIGHT : R_COPY_LEFT;
+        code[e] = (uint8_t)(r << R_PAS_SHIFT);
+      }
     }
   }
 }
