@@ -151,7 +151,7 @@ __device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, fl
 // buffer and the tile is written to the other (ping-pong).  Corners are set
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((NW == 8 && R <= 18) ? 4 : 1))) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((NW == 8 && R <= 14) ? 6 : (NW == 8 && R <= 18) ? 4 : 1))) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
                                                      int nt, int src_role, int dst_role, int x0_role, int coef,
                                                      int nsweeps) {
   constexpr int H = 3 * T, W = 128, RH = NW * R;
@@ -675,7 +675,7 @@ struct RBWork {
 struct BandCfg {
   int nw, r;
 };
-static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}, {8, 18}};
+static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}, {8, 18}, {8, 14}};
 
 static int band_tiles(int S, int tile, int T) {
   const BandCfg b = kBands[tile];
@@ -810,6 +810,7 @@ static void launch_gs_mode(hipStream_t st, const RbGroup& g, int src, int dst, i
   if (g.tile == 1) launch_band_t<MODE, 8, 32>(st, g, src, dst, x0, coef, nsw);
   else if (g.tile == 2) launch_band_t<MODE, 16, 16>(st, g, src, dst, x0, coef, nsw);
   else if (g.tile == 3) launch_band_t<MODE, 8, 18>(st, g, src, dst, x0, coef, nsw);
+  else if (g.tile == 4) launch_band_t<MODE, 8, 14>(st, g, src, dst, x0, coef, nsw);
   else launch_band_t<MODE, 8, 16>(st, g, src, dst, x0, coef, nsw);
 }
 
@@ -971,7 +972,7 @@ void rb_destroy(void* p) {
 }
 
 static bool tiling_ok(int tile, int tsweeps) {
-  return tile >= 0 && tile <= 3 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
+  return tile >= 0 && tile <= 4 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
 }
 
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {
