@@ -634,7 +634,7 @@ extern "C" int eqf_batch_set_band(eqf_batch* b, int tile, int sweeps_per_block) 
   HIPCHK(hipSetDevice(b->ctx->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   if (rb_batch_set_tiling(b->rbg, tile, sweeps_per_block) != EQF_OK)
-    return fail(EQF_EINVAL, "tile must be 0..2 and sweeps 1, 2, 4, 5, 6 or 8");
+    return fail(EQF_EINVAL, "tile must be 0..4 and sweeps 1, 2, 4, 5, 6 or 8");
   return EQF_OK;
 }
 
