@@ -134,3 +134,46 @@ def test_server_red_black_requests_beside_wavefront():
     assert_bit_equal(srv.result(tr[0]), run_oracle(rbs[0], order=oracle.RB).f["density"], "red-black vs oracle")
     assert srv.stats()["scene_frames"] == sum(sc["frames"] for sc in wf + rbs)
     srv.close()
+
+
+def test_server_churn_results_equal_solo_runs():
+    """Rounds of submissions while earlier requests are resident: finished
+    tickets are released (their records freed) and new requests take their
+    place, in both modes and with one red-black iteration count changing
+    between rounds, so the fused launch's scene list and the red-black launch
+    groups change membership every few ticks.  Every result equals its solo
+    run bit for bit."""
+    _case("server churn")
+    rng = np.random.default_rng(33)
+    srv = _abi.Server(hbm_budget_bytes=8 << 30)
+    pending = {}
+    k = 0
+    for rnd in range(4):
+        for _ in range(12):
+            N = int(rng.integers(8, 220))
+            rb = bool(rng.random() < 0.35)
+            sc = scenes.stam_scene(N, scenes.derive_seed(34, k), iters=4 + (rnd % 2 if rb else 0),
+                                   frames=int(rng.integers(1, 7)), n_rects=k % 3)
+            mode = _abi.GS_REDBLACK if rb else _abi.GS_WAVEFRONT
+            pending[srv.submit(sc, gs_mode=mode)] = (sc, mode)
+            k += 1
+        time.sleep(0.05)
+        for t in list(pending):  # collect what has finished, free its record
+            st = srv.poll(t)
+            if st.state == _abi.REQ_DONE:
+                sc, mode = pending.pop(t)
+                s = gpu_scene(sc, mode)
+                s.step(sc["frames"])
+                s.sync()
+                assert_bit_equal(srv.result(t), s.download(_abi.DENSITY), f"churn N={sc['size'] - 2} mode={mode}")
+                s.close()
+                srv.release(t)
+    srv.wait_idle(120000)
+    for t, (sc, mode) in pending.items():
+        assert srv.poll(t).state == _abi.REQ_DONE
+        s = gpu_scene(sc, mode)
+        s.step(sc["frames"])
+        s.sync()
+        assert_bit_equal(srv.result(t), s.download(_abi.DENSITY), f"churn N={sc['size'] - 2} mode={mode}")
+        s.close()
+    srv.close()
