@@ -9,8 +9,11 @@ whole batch.  value = sum(N^2) * K / elapsed (interior cells; the ring is not
 counted).
 
 Also reported beside it: the single-grid drop-in on the same mix (one scene
-at a time), the N=1024 single grid, the HBM roofline of the frame kernel
-and the CPU oracle timed on this host.
+at a time), the N=1024 single grid, the red-black legs, snapshots, serving,
+the HBM roofline of the frame kernel and the CPU leg.  The CPU leg ("cpu") is
+the only part of this script that runs the oracle: it checks the red-black
+legs' GPU outputs against the f64 oracle (their vs_fp64 and Poisson residual
+fields) and times the oracle on this host as cpu_baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--scenes 512]
 """
@@ -29,6 +32,9 @@ import numpy as np  # noqa: E402
 from equilibrium_amd import _abi, scenes  # noqa: E402
 
 METRIC = "cell-updates/s (N=4096 single + batched mix); p50/p99 frame latency; HBM GB/s"
+# GPU outputs the CPU leg checks against the oracle (every oracle call of this
+# script runs in that leg)
+CHECKS = {}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -106,7 +112,6 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
     scenes (band passes over all scenes' tiles with register temporal
     blocking), band shape and sweeps per launch autotuned; accuracy vs the f64
     red-black oracle on one scene per size, over the first frames."""
-    import oracle
     from equilibrium_amd import tune
 
     def mk(sc):
@@ -142,38 +147,48 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
            "sweeps_per_launch": tsw, "roofline_alg_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
                HBM_PEAK_GBS * 1e9)}
     if check_frames > 0:
-        errs, ref_errs = {}, {}
+        # GPU fields after the first frames of one scene per size, checked
+        # against the f64 oracle in the CPU leg
+        got = {}
         for N in sorted({sc["size"] - 2 for sc in scs}):
             sc = [x for x in scs if x["size"] - 2 == N][0]
             g = mk(sc)
             g.step(check_frames)
             g.sync()
-            def oracle_run(dtype, order):
-                st = oracle.State.new(sc["size"]).astype(dtype)
-                for r in sc["rects"]:
-                    st.fill_rect(*r)
-                st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], check_frames, order=order,
-                       src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
-                return st
-
-            def rel(a, b):
-                a = a.astype(np.float64)
-                return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-
-            st = oracle_run(np.float64, oracle.RB)
-            # the reference's own f32 step (lexicographic order) against its f64 version, same scene
-            lex32, lex64 = oracle_run(np.float32, oracle.LEX), oracle_run(np.float64, oracle.LEX)
-            e, eref = {}, {}
-            for name, fid in (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY)):
-                e[name] = rel(g.download(fid), st.f[name])
-                eref[name] = rel(lex32.f[name], lex64.f[name])
+            got[N] = (sc, {name: g.download(fid) for name, fid in
+                           (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY))})
             g.close()
-            errs[f"N={N}"] = e
-            ref_errs[f"N={N}"] = eref
-        res["vs_fp64"] = {"frames": check_frames, "rel_norm": errs,
-                          "worst_rel_norm": max(max(v.values()) for v in errs.values()),
-                          "reference_f32_vs_fp64_rel_norm": ref_errs}
+        CHECKS["rbmix"] = (res, check_frames, got)
     return res
+
+
+def check_batch_redblack(res, check_frames, got):
+    """vs_fp64 of the red-black batch leg: the GPU fields after the first
+    frames against the f64 red-black oracle, beside the reference's own f32
+    step (lexicographic order) against its f64 version on the same scene."""
+    import oracle
+
+    def oracle_run(sc, dtype, order):
+        st = oracle.State.new(sc["size"]).astype(dtype)
+        for r in sc["rects"]:
+            st.fill_rect(*r)
+        st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], check_frames, order=order,
+               src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+        return st
+
+    def rel(a, b):
+        a = a.astype(np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    errs, ref_errs = {}, {}
+    for N, (sc, fields) in got.items():
+        st = oracle_run(sc, np.float64, oracle.RB)
+        lex32, lex64 = oracle_run(sc, np.float32, oracle.LEX), oracle_run(sc, np.float64, oracle.LEX)
+        errs[f"N={N}"] = {name: rel(fields[name], st.f[name]) for name in fields}
+        ref_errs[f"N={N}"] = {name: rel(lex32.f[name], lex64.f[name]) for name in fields}
+    res["vs_fp64"] = {"frames": check_frames, "rel_norm": errs,
+                      "worst_rel_norm": max(max(v.values()) for v in errs.values()),
+                      "reference_f32_vs_fp64_rel_norm": ref_errs}
 
 
 def bench_single_dropin(scs, frames):
@@ -259,42 +274,57 @@ def bench_redblack_4096(steps, check_frames, N=4096, tiling=None):
            "roofline_frac": rate * b_alg(sc["iters"]) / (HBM_PEAK_GBS * 1e9)}
     h.close()
     if check_frames > 0:
-        import oracle
-        from helpers import oracle_state
-
+        # GPU fields after each of the first frames and the last projection's
+        # pressure and divergence, checked against the f64 oracle in the CPU leg
         g = _abi.Scene(scenes.scene_params(sc, _abi.GS_REDBLACK))
         scenes.apply_to_scene(g, sc)
         g.set_tuning(tile, tsw)
-        st = oracle_state(sc).astype(np.float64)
-        srcd = np.ascontiguousarray(sc["src_d"], np.float64)
-        errs = []
-        t_or = time.perf_counter()
+        frames = []
         for f in range(check_frames):
             g.step(1)
             g.sync()
-            st.f["density"] += srcd
-            st.f["scratch"] += srcd
-            st.step_par_f64(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"])
-            e = {}
-            for name, fid in (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY)):
-                got = g.download(fid).astype(np.float64)
-                want = st.f[name]
-                e[name] = {"max_abs": float(np.max(np.abs(got - want))),
-                           "rel_norm": float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))}
-            errs.append(e)
-            log(f"rb N={N} frame {f}: {e}")
-        p_gpu = g.download(_abi.VX0).astype(np.float64)
-        div_gpu = g.download(_abi.VY0).astype(np.float64)
-        mx, rr, rd = oracle.poisson_residual(p_gpu, div_gpu, N + 2)
-        mx64, rr64, rd64 = oracle.poisson_residual(st.f["vx0"], st.f["vy0"], N + 2)
-        res["vs_fp64"] = {"frames": check_frames, "per_frame": errs,
-                          "worst_rel_norm": max(max(x["rel_norm"] for x in e.values()) for e in errs),
-                          "oracle_seconds": time.perf_counter() - t_or}
-        res["poisson_residual"] = {"gpu_max_abs": mx, "gpu_rms": rr, "div_rms": rd,
-                                   "gpu_rms_over_div_rms": rr / max(rd, 1e-300),
-                                   "fp64_rms_over_div_rms": rr64 / max(rd64, 1e-300)}
+            frames.append({name: g.download(fid) for name, fid in
+                           (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY))})
+        p_gpu, div_gpu = g.download(_abi.VX0), g.download(_abi.VY0)
         g.close()
+        CHECKS["rb4096"] = (res, sc, frames, p_gpu, div_gpu)
     return res
+
+
+def check_redblack_4096(res, sc, frames, p_gpu, div_gpu):
+    """vs_fp64 and poisson_residual of the N=4096 leg: per-frame max-abs and
+    relative error of density and velocity against the f64 red-black oracle,
+    and the pressure Poisson residual of the GPU's last projection beside the
+    f64 oracle's."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle
+    from helpers import oracle_state
+
+    N = sc["size"] - 2
+    st = oracle_state(sc).astype(np.float64)
+    srcd = np.ascontiguousarray(sc["src_d"], np.float64)
+    errs = []
+    t_or = time.perf_counter()
+    for f, fields in enumerate(frames):
+        st.f["density"] += srcd
+        st.f["scratch"] += srcd
+        st.step_par_f64(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"])
+        e = {}
+        for name, got in fields.items():
+            got = got.astype(np.float64)
+            want = st.f[name]
+            e[name] = {"max_abs": float(np.max(np.abs(got - want))),
+                       "rel_norm": float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))}
+        errs.append(e)
+        log(f"rb N={N} frame {f}: {e}")
+    mx, rr, rd = oracle.poisson_residual(p_gpu.astype(np.float64), div_gpu.astype(np.float64), N + 2)
+    mx64, rr64, rd64 = oracle.poisson_residual(st.f["vx0"], st.f["vy0"], N + 2)
+    res["vs_fp64"] = {"frames": len(frames), "per_frame": errs,
+                      "worst_rel_norm": max(max(x["rel_norm"] for x in e.values()) for e in errs),
+                      "oracle_seconds": time.perf_counter() - t_or}
+    res["poisson_residual"] = {"gpu_max_abs": mx, "gpu_rms": rr, "div_rms": rd,
+                               "gpu_rms_over_div_rms": rr / max(rd, 1e-300),
+                               "fp64_rms_over_div_rms": rr64 / max(rd64, 1e-300)}
 
 
 def bench_snapshots(frames, depth=2, N=4096, tiling=None):
@@ -563,7 +593,13 @@ def main():
         out["serving"] = bench_serving(rates, args.serve_seconds)
 
     if "cpu" in legs:
-        # CPU baseline: oracle on this host, bounded sample
+        # The CPU leg: the oracle's accuracy checks of the red-black legs, then
+        # the CPU baseline (oracle on this host, bounded sample)
+        if "rbmix" in CHECKS:
+            check_batch_redblack(*CHECKS["rbmix"])
+            log(f"red-black batch vs fp64: {out['batched_mix_redblack']['vs_fp64']}")
+        if "rb4096" in CHECKS:
+            check_redblack_4096(*CHECKS["rb4096"])
         threads = max(1, min(16, os.cpu_count() or 1))
         rate, wcpu, cframes = cpu_baseline(mix, args.cpu_seconds, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
