@@ -123,6 +123,7 @@ def lib():
             "eqf_get_mask": (I, [P, P]),
             "eqf_add_density": (I, [P, U32, U32, F]),
             "eqf_add_velocity": (I, [P, U32, U32, F, F]),
+            "eqf_add_cell": (I, [P, I, U32, U32, F]),
             "eqf_add_noise": (I, [P, F, U32, U32]),
             "eqf_add_source": (I, [P, I, P]),
             "eqf_set_frame_source": (I, [P, I, P]),
@@ -261,6 +262,9 @@ class Scene:
 
     def add_velocity(self, x, y, ax, ay):
         check(lib().eqf_add_velocity(self.h, x, y, ax, ay))
+
+    def add_cell(self, field, x, y, amount):
+        check(lib().eqf_add_cell(self.h, field, x, y, amount))
 
     def add_noise(self, angle_degrees, rand_x, rand_y):
         check(lib().eqf_add_noise(self.h, angle_degrees, rand_x, rand_y))

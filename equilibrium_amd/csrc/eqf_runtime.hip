@@ -434,6 +434,18 @@ extern "C" int eqf_add_velocity(eqf_scene* s, uint32_t x, uint32_t y, float ax, 
   return EQF_OK;
 }
 
+extern "C" int eqf_add_cell(eqf_scene* s, int field, uint32_t x, uint32_t y, float amount) {
+  if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (field == EQF_DENSITY) return eqf_add_density(s, x, y, amount);
+  if (field != EQF_VX && field != EQF_VY) return fail(EQF_EINVAL, "field must be EQF_DENSITY, EQF_VX or EQF_VY");
+  // one component only: adding 0 to the other would turn its -0 into +0
+  const int64_t o = clamp_cell(s, x, y);
+  hipLaunchKernelGGL(eqf_add_cell_kernel, dim3(1), dim3(1), 0, s->stream,
+                     field_ptr(s, field == EQF_VX ? F_U : F_V) + o, (float*)nullptr, amount, 0.0f);
+  HIPCHK(hipGetLastError());
+  return EQF_OK;
+}
+
 extern "C" int eqf_add_noise(eqf_scene* s, float angle_degrees, uint32_t rand_x, uint32_t rand_y) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
   // rotate (rand_x, rand_y) about the centre by angle_degrees, in f32

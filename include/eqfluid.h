@@ -107,6 +107,10 @@ int eqf_get_mask(const eqf_scene *s, uint8_t *mask);
  * clamped index; density is added to the density and scratch fields. */
 int eqf_add_density(eqf_scene *s, uint32_t x, uint32_t y, float amount);
 int eqf_add_velocity(eqf_scene *s, uint32_t x, uint32_t y, float ax, float ay);
+/* One cell of one field, with eqf_add_source's field meaning: EQF_DENSITY is
+ * add_density(x, y, amount); EQF_VX / EQF_VY add to that velocity component
+ * only (add_velocity with the other component untouched). */
+int eqf_add_cell(eqf_scene *s, int field, uint32_t x, uint32_t y, float amount);
 /* Fluid::add_noise (fluid.rs:575-599) with its random inputs supplied by the
  * caller: the line from the centre cell (size/2, size/2) to (rand_x, rand_y)
  * is rotated by `angle_degrees` about the centre, and twice the rotated end

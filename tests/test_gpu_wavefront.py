@@ -194,6 +194,35 @@ def test_batch_mixed_sizes_equals_solo():
         assert_state_equal(download_all(h), st, f"batch N={sc['size'] - 2}")
 
 
+@pytest.mark.parametrize("mode", [_abi.GS_WAVEFRONT, _abi.GS_REDBLACK])
+def test_add_cell_touches_one_cell_of_its_fields(mode):
+    """eqf_add_cell: EQF_DENSITY adds to density and scratch like add_density;
+    EQF_VX / EQF_VY add to that component only (the other keeps its -0);
+    indices clamp like idx! (fluid.rs:31-35)."""
+    N = 70
+    S = N + 2
+    s = gpu_scene(scenes.stam_scene(N, scenes.derive_seed(60), iters=3, frames=1, n_rects=1), mode)
+    neg0 = np.full(S * S, -0.0, np.float32)
+    s.upload(_abi.VX, neg0)
+    s.upload(_abi.VY, neg0)
+    before = download_all(s)
+    s.add_cell(_abi.DENSITY, 5, 9, 2.5)
+    s.add_cell(_abi.VX, 10, 3, 1.25)
+    s.add_cell(_abi.VY, 10 ** 6, 4, -0.75)  # x clamps to S-1
+    s.sync()
+    after = download_all(s)
+    want = {k: v.copy() for k, v in before.items()}
+    for name in ("density", "scratch"):
+        want[name][5 + 9 * S] = np.float32(want[name][5 + 9 * S] + np.float32(2.5))
+    want["vx"][10 + 3 * S] = np.float32(np.float32(-0.0) + np.float32(1.25))
+    want["vy"][(S - 1) + 4 * S] = np.float32(np.float32(-0.0) + np.float32(-0.75))
+    for name in FIELD_NAMES:
+        assert_bit_equal(after[name], want[name], name)
+    with pytest.raises(_abi.EqfError):
+        s.add_cell(_abi.VX0, 1, 1, 1.0)
+    s.close()
+
+
 def test_add_noise_rotation_closed_forms():
     """add_noise (fluid.rs:575-599) with the random inputs supplied: at 0, 90
     and 180 degrees the rotated end point has a closed form (the centre is
