@@ -208,10 +208,49 @@ def bench_single_dropin(scs, frames):
     return wall
 
 
+def cpu_single_1024(frames=3):
+    """The oracle's exact (lexicographic f32) step on the N=1024 grid on one
+    core, as the reference runs it: one warm-up frame, then `frames` frames
+    timed one by one."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import oracle_state
+
+    sc = scenes.single_1024()
+    st = oracle_state(sc)
+
+    def frame():
+        st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], 1,
+               src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+
+    frame()
+    times = []
+    for _ in range(frames):
+        t0 = time.perf_counter()
+        frame()
+        times.append(time.perf_counter() - t0)
+    N = sc["size"] - 2
+    med = sorted(times)[len(times) // 2]
+    return {"value": N * N / med, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "median_frame_s": med, "total_s": sum(times),
+            "sample": f"N={N} exact step, 1 warm-up + {frames} timed frames on one core"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scs, seconds, threads):
     """The CPU oracle (C restatement of the reference step, -O2, no FMA) on the
-    same scenes, one scene per thread, whole frames of the mix until about
-    `seconds` of wall time have passed (at least one frame)."""
+    same scenes, one scene per thread: one untimed warm-up frame of the mix,
+    then whole frames until about `seconds` of wall time have passed (at
+    least one frame)."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle  # noqa: F401
     from helpers import oracle_state
@@ -225,8 +264,9 @@ def cpu_baseline(scs, seconds, threads):
 
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     frames = 0
-    t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(run, range(len(scs))))  # warm-up frame, untimed
+        t0 = time.perf_counter()
         while True:
             list(ex.map(run, range(len(scs))))
             frames += 1
@@ -603,10 +643,12 @@ def main():
         threads = max(1, min(16, os.cpu_count() or 1))
         rate, wcpu, cframes = cpu_baseline(mix, args.cpu_seconds, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
-                               "kind": "port",
+                               "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                                "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) from the "
                                          f"initial state, one scene per thread, {wcpu:.1f} s"}
         log(f"cpu oracle: {rate:.4g} cell-updates/s on {threads} threads ({wcpu:.1f} s)")
+        out["cpu_single_n1024"] = cpu_single_1024()
+        log(f"cpu oracle N=1024 on one core: {out['cpu_single_n1024']}")
 
     print(json.dumps(out), flush=True)
 
