@@ -435,7 +435,7 @@ def bench_serving(rates, seconds, budget_gb=200.0, n_max=2048, buckets=16):
     against an HBM budget; each rate runs `seconds` of arrivals, then the
     remaining requests are cancelled.  Scenes up to N=1024 run in exact
     wavefront mode (one fused launch per tick), larger ones in red-black mode
-    (their frame graphs beside it).  p50/p99 frame latency = tick durations
+    (launch groups beside it).  p50/p99 frame latency = tick durations
     weighted by resident scenes.  To keep the client on the Poisson schedule,
     N is rounded to one of `buckets` log-spaced sizes whose scenes (sources,
     obstacles) are generated before the clock starts and lent to the server
@@ -500,7 +500,7 @@ def main():
                     help="band:sweeps for the red-black batched mix instead of autotuning (e.g. 3:6)")
     ap.add_argument("--snap-frames", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
-    ap.add_argument("--serve-rates", type=str, default="50,500,2000")
+    ap.add_argument("--serve-rates", type=str, default="50,100,200,500,1000,2000")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -644,8 +644,8 @@ def main():
         rate, wcpu, cframes = cpu_baseline(mix, args.cpu_seconds, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
-                               "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) from the "
-                                         f"initial state, one scene per thread, {wcpu:.1f} s"}
+                               "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) after one untimed "
+                                         f"warm-up frame, one scene per thread, {wcpu:.1f} s"}
         log(f"cpu oracle: {rate:.4g} cell-updates/s on {threads} threads ({wcpu:.1f} s)")
         out["cpu_single_n1024"] = cpu_single_1024()
         log(f"cpu oracle N=1024 on one core: {out['cpu_single_n1024']}")
