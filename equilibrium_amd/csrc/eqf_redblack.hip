@@ -657,7 +657,7 @@ struct RbGroup {
   int n = 0, maxN = 0, tiles = 0, iblocks = 0, cblocks = 0, bpp = 0, brc = 0, bp = 0;
   bool any_src = false;
   int diff_iters = 0, press_iters = 0;
-  int tile = 0, tsweeps = 4;  // band shape index (kBands) and sweeps per launch
+  int tile = 4, tsweeps = 5;  // band shape index (kBands) and sweeps per launch (untuned default)
   int tab_tile = -1, tab_T = -1;
   hipGraphExec_t graph = nullptr;
   int64_t graph_frames = 0;
