@@ -448,12 +448,17 @@ extern "C" int eqf_add_cell(eqf_scene* s, int field, uint32_t x, uint32_t y, flo
 
 extern "C" int eqf_add_noise(eqf_scene* s, float angle_degrees, uint32_t rand_x, uint32_t rand_y) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
-  // rotate (rand_x, rand_y) about the centre by angle_degrees, in f32
+  // Rotate the line's end point (rand_x, rand_y) about the centre by
+  // angle_degrees, all in f32 like the reference's f32 line string
+  // (fluid.rs:586-590): geo 0.18 `rotate_around_point` (absent here; its
+  // published rotation restated) turns degrees into radians with f32
+  // `to_radians` (angle * (PI / 180)), takes f32 sin and cos, and maps
+  // (x, y) to ((x - x0) cos - (y - y0) sin + x0, (x - x0) sin + (y - y0) cos + y0).
   const float c = (float)(s->S / 2);
-  const double th = (double)angle_degrees * 3.14159265358979323846 / 180.0;
-  const float cs = (float)std::cos(th), sn = (float)std::sin(th);
+  const float rad = angle_degrees * (3.14159265358979323846f / 180.0f);
+  const float cs = std::cos(rad), sn = std::sin(rad);
   const float dx = (float)rand_x - c, dy = (float)rand_y - c;
-  const float rx = c + (dx * cs - dy * sn), ry = c + (dx * sn + dy * cs);
+  const float rx = (dx * cs - dy * sn) + c, ry = (dx * sn + dy * cs) + c;
   return eqf_add_velocity(s, (uint32_t)(s->S / 2), (uint32_t)(s->S / 2), rx * 2.0f, ry * 2.0f);
 }
 
