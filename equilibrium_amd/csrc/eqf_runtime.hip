@@ -345,6 +345,7 @@ extern "C" int eqf_scene_get_params(const eqf_scene* s, eqf_params* out) {
 
 extern "C" int eqf_scene_set_params(eqf_scene* s, const eqf_params* p) {
   if (!s || !p) return fail(EQF_EINVAL, "NULL argument");
+  if (s->in_batch) return fail(EQF_EBUSY, "scene is part of a batch");  // the batch holds its descriptor
   if (p->size != s->p.size || p->gs_mode != s->p.gs_mode)
     return fail(EQF_EINVAL, "size and gs_mode are fixed for the lifetime of a scene");
   int rc = check_params(p);

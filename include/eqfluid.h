@@ -149,8 +149,9 @@ int eqf_scene_set_tuning(eqf_scene *s, int tile, int sweeps_per_block);
  * fused launch per frame runs every scene's whole frame.  Red-black mode (any
  * sizes, one iteration count): every kernel of the frame covers all scenes
  * (band passes over all scenes' tiles, then the grid-wide phases).  Scenes
- * stay owned by the caller but must not be stepped separately while in a
- * batch. */
+ * stay owned by the caller; while in a batch they must not be stepped
+ * separately, and their mask, parameters and frame sources are fixed
+ * (EQF_EBUSY). */
 int eqf_batch_create(eqf_ctx *ctx, eqf_scene *const *scenes, int n, eqf_batch **out);
 int eqf_batch_step(eqf_batch *b, int64_t nframes);
 int eqf_batch_sync(eqf_batch *b);

@@ -160,4 +160,11 @@ def test_redblack_batch_rejects_mixed_modes_and_counts():
         bt.set_tuning(8, 0)  # wavefront knob
     with pytest.raises(_abi.EqfError):
         bt.set_band(0, 3)
+    # a batch member's descriptor is held by the batch: no edits while in it
+    pa = scenes.scene_params(scenes.stam_scene(40, 1, iters=4, frames=1), _abi.GS_REDBLACK)
+    for edit in (lambda: a.fill_rect(2, 2, 5, 5), lambda: a.step(1), lambda: a.set_params(pa)):
+        with pytest.raises(_abi.EqfError) as ei:
+            edit()
+        assert ei.value.code == _abi.EQF_EBUSY
     bt.close()
+    a.fill_rect(2, 2, 5, 5)  # free again once the batch is closed
