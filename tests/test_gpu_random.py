@@ -59,3 +59,29 @@ def test_random_scenes_batched_equal_oracle():
     b.close()
     for k, (sc, h) in enumerate(zip(scs, hs)):
         assert_state_equal(download_all(h), run_oracle(sc, frames=frames), f"batched random k={k + 24}")
+
+
+def test_exact_batch_of_strip_boundary_sizes_equals_oracle():
+    """One fused launch over scenes at and around the strip (64 rows) and
+    period (16 slots) boundaries, down to size 3, with sweep counts from 3 to
+    6 and frame sources on some fields only; every scene equal to the oracle
+    bit for bit."""
+    sizes = [1, 2, 3, 5, 15, 16, 17, 62, 63, 64, 65, 66, 126, 127, 128, 129, 130, 191, 192, 193, 255, 257]
+    scs = []
+    for k, N in enumerate(sizes):
+        sc = scenes.stam_scene(N, scenes.derive_seed(52, k), iters=3 + k % 4, frames=2, n_rects=k % 4,
+                               viscosity=10 ** (-3 - k % 3), diffusion=10 ** (-5 + k % 2), dt=0.05 + 0.1 * (k % 3))
+        if k % 3 == 1:
+            sc["src_u"] = None
+        if k % 5 == 2:
+            sc["src_d"] = None
+        scs.append(sc)
+    print(f"[case] exact batch of {len(scs)} boundary sizes", flush=True)
+    hs = [gpu_scene(sc) for sc in scs]
+    b = _abi.Batch(hs)
+    b.step(2)
+    b.sync()
+    b.close()
+    for sc, h in zip(scs, hs):
+        assert_state_equal(download_all(h), run_oracle(sc, frames=2), f"exact batch N={sc['size'] - 2}")
+        h.close()
