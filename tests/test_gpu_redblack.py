@@ -36,7 +36,7 @@ def test_redblack_bit_exact_all_tilings(tile, tsw):
 
 
 @pytest.mark.parametrize("N,iters,frames", [(1, 2, 2), (2, 3, 2), (3, 3, 2), (8, 3, 2), (30, 5, 3), (31, 4, 2),
-                                           (33, 4, 2), (64, 20, 2), (201, 9, 2)])
+                                           (33, 4, 2), (64, 20, 2), (201, 9, 2), (1100, 5, 1), (2047, 4, 1)])
 def test_redblack_bit_exact_sizes(N, iters, frames):
     _case(f"rb N={N}")
     sc = scenes.stam_scene(N, scenes.derive_seed(21, N), iters=iters, frames=frames, n_rects=3)
