@@ -8,6 +8,7 @@ All arrays are numpy, row-major ``idx(x, y) = x + y*size`` (fluid.rs:31-35),
 float32 for the *_f32 functions and float64 for the *_f64 ones; the mask is
 uint8 with 0 = ContainerWall::DefaultWall and 1 = ContainerWall::NoWall.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -30,36 +31,57 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
 
 
+def load_library(path):
+    """ctypes handle on a build of eqf_oracle.c at `path`, with every entry
+    point's argument types declared."""
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    for sfx, F in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
+        getattr(L, "ora_set_bnd" + sfx).argtypes = [ctypes.c_int, P, I64, P]
+        getattr(L, "ora_lin_solve" + sfx).argtypes = [ctypes.c_int, P, P, F, F, I64, I64, P, ctypes.c_int]
+        getattr(L, "ora_diffuse" + sfx).argtypes = [ctypes.c_int, P, P, F, I64, F, I64, P, ctypes.c_int]
+        getattr(L, "ora_project" + sfx).argtypes = [P, P, P, P, I64, I64, P, ctypes.c_int]
+        getattr(L, "ora_advect" + sfx).argtypes = [ctypes.c_int, P, P, P, P, I64, F, P]
+        getattr(L, "ora_step" + sfx).argtypes = [P] * 6 + [I64, F, F, F, I64, I64, P, ctypes.c_int]
+        getattr(L, "ora_add_sources" + sfx).argtypes = [P] * 7 + [I64]
+        getattr(L, "ora_run" + sfx).argtypes = [P] * 6 + [I64, F, F, F, I64, I64, P, ctypes.c_int, P, P, P, I64]
+    L.ora_init_f32.argtypes = [P] * 7 + [ctypes.c_uint32, ctypes.c_int]
+    L.ora_fill_rect.argtypes = [P, ctypes.c_uint32, I64, I64, I64, I64]
+    L.ora_rect_valid.argtypes = [I64] * 5
+    L.ora_rect_valid.restype = ctypes.c_int
+    L.ora_to_coordinate.argtypes = [I64, I64, P, P]
+    L.ora_idx.argtypes = [I64, I64, I64]
+    L.ora_idx.restype = I64
+    L.ora_step_par_f64.argtypes = [P] * 6 + [I64, ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, I64, P]
+    L.ora_poisson_residual_f64.argtypes = [P, P, I64, P, P]
+    L.ora_poisson_residual_f64.restype = ctypes.c_double
+    L.ora_render_rgba.argtypes = [P, P, I64, P, P, P, P]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing; run `make -C oracle`")
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.c_void_p
-        I64 = ctypes.c_int64
-        for sfx, F in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
-            getattr(L, "ora_set_bnd" + sfx).argtypes = [ctypes.c_int, P, I64, P]
-            getattr(L, "ora_lin_solve" + sfx).argtypes = [ctypes.c_int, P, P, F, F, I64, I64, P, ctypes.c_int]
-            getattr(L, "ora_diffuse" + sfx).argtypes = [ctypes.c_int, P, P, F, I64, F, I64, P, ctypes.c_int]
-            getattr(L, "ora_project" + sfx).argtypes = [P, P, P, P, I64, I64, P, ctypes.c_int]
-            getattr(L, "ora_advect" + sfx).argtypes = [ctypes.c_int, P, P, P, P, I64, F, P]
-            getattr(L, "ora_step" + sfx).argtypes = [P] * 6 + [I64, F, F, F, I64, I64, P, ctypes.c_int]
-            getattr(L, "ora_add_sources" + sfx).argtypes = [P] * 7 + [I64]
-            getattr(L, "ora_run" + sfx).argtypes = [P] * 6 + [I64, F, F, F, I64, I64, P, ctypes.c_int, P, P, P, I64]
-        L.ora_init_f32.argtypes = [P] * 7 + [ctypes.c_uint32, ctypes.c_int]
-        L.ora_fill_rect.argtypes = [P, ctypes.c_uint32, I64, I64, I64, I64]
-        L.ora_rect_valid.argtypes = [I64] * 5
-        L.ora_rect_valid.restype = ctypes.c_int
-        L.ora_to_coordinate.argtypes = [I64, I64, P, P]
-        L.ora_idx.argtypes = [I64, I64, I64]
-        L.ora_idx.restype = I64
-        L.ora_step_par_f64.argtypes = [P] * 6 + [I64, ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, I64, P]
-        L.ora_poisson_residual_f64.argtypes = [P, P, I64, P, P]
-        L.ora_poisson_residual_f64.restype = ctypes.c_double
-        L.ora_render_rgba.argtypes = [P, P, I64, P, P, P, P]
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(handle):
+    """Routes every function of this module to another build of the oracle
+    (a handle from load_library) inside the `with` block.  Used by the
+    mutation tests, which check that planted mistakes fail the known-answer
+    tests."""
+    global _lib
+    saved = lib()
+    _lib = handle
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _p(a):
