@@ -38,6 +38,13 @@ struct eqf_scene {
   eqf::SceneDesc desc{};
   std::vector<float> hbuf;  // layout staging
   int in_batch = 0;
+  // Batch membership: the batch advances the scene on its own stream.  Work
+  // queued on the scene's stream meanwhile first waits for the batch's last
+  // step (eqf_internal_scene_enter), and the batch's next step waits for it
+  // (ev_ops, recorded when ops_pending).
+  struct eqf_batch* batch = nullptr;
+  hipEvent_t ev_ops = nullptr;
+  bool ops_pending = false;
   bool owns_stream = true;
   void* rb = nullptr;       // red-black workspace (eqf_redblack.hip)
 };
@@ -54,6 +61,9 @@ int eqf_scene_waves(int ns, int P);
 hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int waves, int lds_floats, int nframes,
                                hipStream_t st);
 int eqf_internal_error(int code, const std::string& msg);
+// Orders work about to be queued on the scene's stream after the last step of
+// the scene's batch (no-op outside a batch).
+int eqf_internal_scene_enter(eqf_scene* s);
 int eqf_internal_check_params(const eqf_params* p);
 // eqf_scene_create on a caller-owned stream (the server's admission stream).
 int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, eqf_scene** out);

@@ -687,14 +687,16 @@ static uint64_t group_key(const RbGroup& g) {
   uint64_t key = 1469598103934665603ull;
   auto mix = [&key](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
   for (const eqf_scene* s : g.scenes) {
-    const SceneDesc& d = s->desc;
+    // The tables embed a copy of every descriptor, so every byte of it is
+    // part of the key (dt0 included: with zero viscosity and diffusion a new
+    // delta_t changes only dt0).  The descriptor is value-initialised, so
+    // its padding bytes are zero.
+    SceneDesc d = s->desc;
+    d.prio = 0;
+    uint64_t words[(sizeof(SceneDesc) + 7) / 8] = {};
+    std::memcpy(words, &d, sizeof(SceneDesc));
     mix(s->uid);
-    mix((uint64_t)d.diff_iters);
-    mix((uint64_t)d.press_iters);
-    mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_visc));
-    mix((uint64_t)__builtin_bit_cast(uint32_t, d.a_diff));
-    mix((uint64_t)d.has_src);
-    mix((uint64_t)d.nbnd[0] | ((uint64_t)d.nbnd[1] << 21) | ((uint64_t)d.nbnd[2] << 42));
+    for (uint64_t w : words) mix(w);
   }
   mix((uint64_t)g.tile * 16 + (uint64_t)g.tsweeps);
   return key;

@@ -100,7 +100,7 @@ extern "C" void eqf_colors_default(eqf_colors* c) {
 
 extern "C" int eqf_render_rgba(eqf_scene* s, const eqf_colors* c, uint8_t* rgba_host) {
   if (!s || !rgba_host) return eqf_internal_error(EQF_EINVAL, "bad arguments");
-  if (s->in_batch) return eqf_internal_error(EQF_EBUSY, "scene is part of a batch");
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   eqf_colors def;
   if (!c) {
     eqf_colors_default(&def);
@@ -156,7 +156,6 @@ static void snapshot_free(eqf_snapshot* sn) {
 
 extern "C" int eqf_snapshot_create(eqf_scene* s, int depth, int with_rgba, const eqf_colors* c, eqf_snapshot** out) {
   if (!s || !out || depth < 1 || depth > 8) return eqf_internal_error(EQF_EINVAL, "bad arguments (depth 1..8)");
-  if (s->in_batch) return eqf_internal_error(EQF_EBUSY, "scene is part of a batch");
   hipError_t e = hipSetDevice(s->ctx->device);
   auto* sn = new eqf_snapshot;
   sn->scene = s;
@@ -191,7 +190,7 @@ extern "C" int eqf_snapshot_capture(eqf_snapshot* sn, int64_t frame) {
   if (!sn) return eqf_internal_error(EQF_EINVAL, "snapshot is NULL");
   if (sn->count == sn->depth) return eqf_internal_error(EQF_EBUSY, "every snapshot slot holds an unreleased capture");
   eqf_scene* s = sn->scene;
-  if (s->in_batch) return eqf_internal_error(EQF_EBUSY, "scene is part of a batch");
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, hipGetErrorString(e));
   auto& sl = sn->slots[(sn->head + sn->count) % sn->depth];

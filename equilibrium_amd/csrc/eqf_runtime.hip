@@ -197,6 +197,7 @@ extern "C" int eqf_ctx_destroy(eqf_ctx* ctx) {
 static void scene_free(eqf_scene* s) {
   if (!s) return;
   if (s->rb) rb_destroy(s->rb);
+  if (s->ev_ops) (void)hipEventDestroy(s->ev_ops);
   (void)hipFree(s->d_main);
   (void)hipFree(s->d_src);
   (void)hipFree(s->d_bnd);
@@ -297,8 +298,10 @@ int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t str
 
 extern "C" int eqf_scene_clone(const eqf_scene* src, eqf_scene** out) {
   if (!src || !out) return fail(EQF_EINVAL, "src/out is NULL");
+  int rc = eqf_internal_scene_enter(const_cast<eqf_scene*>(src));
+  if (rc) return rc;
   eqf_scene* s = nullptr;
-  int rc = scene_alloc(src->ctx, &src->p, &s);
+  rc = scene_alloc(src->ctx, &src->p, &s);
   if (rc) return rc;
   s->mask = src->mask;
   s->bnd_dirty = true;
@@ -419,6 +422,7 @@ static int64_t clamp_cell(const eqf_scene* s, uint32_t x, uint32_t y) {
 
 extern "C" int eqf_add_density(eqf_scene* s, uint32_t x, uint32_t y, float amount) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   const int64_t o = clamp_cell(s, x, y);
   hipLaunchKernelGGL(eqf_add_cell_kernel, dim3(1), dim3(1), 0, s->stream, field_ptr(s, F_D) + o,
                      field_ptr(s, F_SC) + o, amount, amount);
@@ -428,6 +432,7 @@ extern "C" int eqf_add_density(eqf_scene* s, uint32_t x, uint32_t y, float amoun
 
 extern "C" int eqf_add_velocity(eqf_scene* s, uint32_t x, uint32_t y, float ax, float ay) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   const int64_t o = clamp_cell(s, x, y);
   hipLaunchKernelGGL(eqf_add_cell_kernel, dim3(1), dim3(1), 0, s->stream, field_ptr(s, F_U) + o,
                      field_ptr(s, F_V) + o, ax, ay);
@@ -440,6 +445,7 @@ extern "C" int eqf_add_cell(eqf_scene* s, int field, uint32_t x, uint32_t y, flo
   if (field == EQF_DENSITY) return eqf_add_density(s, x, y, amount);
   if (field != EQF_VX && field != EQF_VY) return fail(EQF_EINVAL, "field must be EQF_DENSITY, EQF_VX or EQF_VY");
   // one component only: adding 0 to the other would turn its -0 into +0
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   const int64_t o = clamp_cell(s, x, y);
   hipLaunchKernelGGL(eqf_add_cell_kernel, dim3(1), dim3(1), 0, s->stream,
                      field_ptr(s, field == EQF_VX ? F_U : F_V) + o, (float*)nullptr, amount, 0.0f);
@@ -467,6 +473,7 @@ extern "C" int eqf_add_source(eqf_scene* s, int field, const float* amounts) {
   if (!s || !amounts) return fail(EQF_EINVAL, "NULL argument");
   if (field != EQF_DENSITY && field != EQF_VX && field != EQF_VY)
     return fail(EQF_EINVAL, "sources exist for EQF_DENSITY, EQF_VX and EQF_VY");
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   return eqf_dev_upload(s, field_ptr(s, field), field == EQF_DENSITY ? field_ptr(s, F_SC) : nullptr, amounts, 1);
 }
 
@@ -496,6 +503,7 @@ extern "C" int eqf_set_frame_source(eqf_scene* s, int field, const float* amount
 extern "C" int eqf_upload(eqf_scene* s, int field, const float* host) {
   if (!s || !host) return fail(EQF_EINVAL, "NULL argument");
   int rc = check_field(field);
+  if (!rc) rc = eqf_internal_scene_enter(s);
   if (rc) return rc;
   return upload_rm(s, field_ptr(s, field), host);
 }
@@ -503,6 +511,7 @@ extern "C" int eqf_upload(eqf_scene* s, int field, const float* host) {
 extern "C" int eqf_download(eqf_scene* s, int field, float* host) {
   if (!s || !host) return fail(EQF_EINVAL, "NULL argument");
   int rc = check_field(field);
+  if (!rc) rc = eqf_internal_scene_enter(s);
   if (rc) return rc;
   return download_rm(s, field_ptr(s, field), host);
 }
@@ -546,6 +555,7 @@ extern "C" int eqf_scene_set_tuning(eqf_scene* s, int tile, int sweeps_per_block
 
 extern "C" int eqf_sync(eqf_scene* s) {
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
+  if (int rc = eqf_internal_scene_enter(s)) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   return EQF_OK;
 }
@@ -600,6 +610,13 @@ static int batch_layout(eqf_batch* b) {
 
 extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, eqf_batch** out) {
   if (!ctx || !scenes || n <= 0 || !out) return fail(EQF_EINVAL, "bad arguments");
+  {
+    // a scene listed twice would be advanced twice per frame by concurrent workgroups
+    std::vector<const eqf_scene*> sorted(scenes, scenes + n);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      return fail(EQF_EINVAL, "a scene appears more than once in the batch");
+  }
   for (int i = 0; i < n; ++i) {
     if (!scenes[i]) return fail(EQF_EINVAL, "NULL scene");
     if (scenes[i]->layout != scenes[0]->layout)
@@ -641,7 +658,11 @@ extern "C" int eqf_batch_create(eqf_ctx* ctx, eqf_scene* const* scenes, int n, e
     delete b;
     return rc;
   }
-  for (auto* s : b->scenes) s->in_batch = 1;
+  for (auto* s : b->scenes) {
+    s->in_batch = 1;
+    s->batch = b;
+    s->ops_pending = false;
+  }
   *out = b;
   return EQF_OK;
 }
@@ -697,6 +718,15 @@ int eqf_batch_prio(int ns, int maxns) {
 }
 
 int eqf_internal_prepare(eqf_scene* s) { return sync_desc(s); }
+
+int eqf_internal_scene_enter(eqf_scene* s) {
+  if (!s->batch) return EQF_OK;
+  // ev1 is recorded after every step of the batch (an event never recorded
+  // counts as complete)
+  HIPCHK(hipStreamWaitEvent(s->stream, s->batch->ev1, 0));
+  s->ops_pending = true;
+  return EQF_OK;
+}
 int eqf_internal_error(int code, const std::string& msg) { return fail(code, msg); }
 int eqf_internal_check_params(const eqf_params* p) { return check_params(p); }
 
@@ -711,6 +741,15 @@ extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {
   if (nframes < 0) return fail(EQF_EINVAL, "nframes < 0");
   if (nframes == 0) return EQF_OK;
   HIPCHK(hipSetDevice(b->ctx->device));
+  // work the caller queued on member scenes since the last step (sources,
+  // uploads) runs before this step
+  for (eqf_scene* s : b->scenes) {
+    if (!s->ops_pending) continue;
+    if (!s->ev_ops) HIPCHK(hipEventCreateWithFlags(&s->ev_ops, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(s->ev_ops, s->stream));
+    HIPCHK(hipStreamWaitEvent(b->stream, s->ev_ops, 0));
+    s->ops_pending = false;
+  }
   if (b->rbg) {  // red-black: the group's frames, captured once per nframes
     HIPCHK(hipEventRecord(b->ev0, b->stream));
     std::string err;
@@ -721,7 +760,10 @@ extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {
   }
   // One launch per frame, captured once as a graph of `nframes` launches.
   if (!b->gexec || b->graph_frames != nframes) {
-    if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
+    if (b->gexec) {
+      HIPCHK(hipStreamSynchronize(b->stream));  // the previous replay may still run
+      HIPCHK(hipGraphExecDestroy(b->gexec));
+    }
     b->gexec = nullptr;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -757,9 +799,15 @@ extern "C" int eqf_batch_last_ms(const eqf_batch* b, double* ms) {
 extern "C" int eqf_batch_destroy(eqf_batch* b) {
   if (!b) return EQF_OK;
   (void)hipStreamSynchronize(b->stream);
+  for (auto* s : b->scenes)  // work that waits on the batch's events ends before they go
+    if (s->ops_pending) (void)hipStreamSynchronize(s->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   rb_batch_destroy(b->rbg);
-  for (auto* s : b->scenes) s->in_batch = 0;
+  for (auto* s : b->scenes) {
+    s->in_batch = 0;
+    s->batch = nullptr;
+    s->ops_pending = false;
+  }
   (void)hipFree(b->d_descs);
   (void)hipEventDestroy(b->ev0);
   (void)hipEventDestroy(b->ev1);

@@ -6,9 +6,9 @@ Names and argument meanings follow the Rust items they mirror:
 * ``SimulationConfigs`` -- configs.rs:5-33
 * ``FluidConfigs``      -- configs.rs:37-60 (colours kept as plain tuples)
 * ``ContainerWall``     -- fluid.rs:12-17
-* ``Rectangle``         -- obstacle.rs:89-154 (invalid input raises
+* ``Rectangle``         -- obstacle.rs:29-98 (invalid input raises
   ``ValueError("Invalid input for Rectangle")`` where the reference panics)
-* ``ObstaclesType``     -- obstacle.rs:72-83
+* ``ObstaclesType``     -- obstacle.rs:12-27
 * ``Fluid``             -- fluid.rs:51-620; pub fields become properties that
   read the device-resident state, ``step`` runs on the GPU.
 * ``CurrentSimulation`` -- renderer_helpers.rs:29-81 frame loop (without the

@@ -42,7 +42,7 @@ struct FluidConfigs {
 
 using Point = std::pair<int64_t, int64_t>;  // line_drawing::Point<i64>
 
-// obstacle.rs:89-154
+// obstacle.rs:29-98
 class Rectangle {
  public:
   Point down_left_point, up_right_point;
@@ -61,7 +61,7 @@ class Rectangle {
   std::vector<Point> approximate_points_;
 };
 
-// obstacle.rs:72-83 (one variant)
+// obstacle.rs:12-27 (one variant)
 struct ObstaclesType {
   Rectangle rectangle;
   std::vector<Point>& get_approximate_points() { return rectangle.get_approximate_points(); }

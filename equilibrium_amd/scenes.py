@@ -57,8 +57,9 @@ def derive_seed(*keys):
 
 
 def gaussian(size, cx, cy, sigma, amp):
-    y, x = np.mgrid[0:size, 0:size].astype(np.float64)
-    g = amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2.0 * sigma * sigma))
+    c = np.arange(size, dtype=np.float64)
+    r2 = ((c - cx) ** 2)[None, :] + ((c - cy) ** 2)[:, None]   # [y, x]
+    g = amp * np.exp(-r2 / (2.0 * sigma * sigma))
     return g.astype(np.float32).ravel()
 
 
@@ -137,11 +138,18 @@ def single_1024(seed=None):
 
 
 def curl_noise(S, seed, smooth, peak):
-    """Divergence-free velocity: curl of Gaussian-smoothed seeded noise."""
-    from scipy.ndimage import gaussian_filter
+    """Divergence-free velocity: curl of Gaussian-smoothed seeded noise.  The
+    smoothing is periodic (a product with the Gaussian's transfer function
+    exp(-2 pi^2 sigma^2 |k|^2) in the Fourier domain, on the next FFT-friendly
+    size), about a second at S = 4098."""
+    import scipy.fft
 
     g = np.random.Generator(np.random.PCG64(seed & MASK64))
-    psi = gaussian_filter(g.standard_normal((S, S)), smooth, mode="wrap")
+    L = scipy.fft.next_fast_len(S, real=True)   # noise on an L x L torus, cropped to S x S
+    k = np.fft.fftfreq(L)
+    kr = np.fft.rfftfreq(L)
+    h = np.exp(-2.0 * np.pi ** 2 * smooth ** 2 * (k[:, None] ** 2 + kr[None, :] ** 2))
+    psi = scipy.fft.irfft2(scipy.fft.rfft2(g.standard_normal((L, L))) * h, s=(L, L))[:S, :S]
     u = np.zeros((S, S))
     v = np.zeros((S, S))
     u[1:-1, :] = (psi[2:, :] - psi[:-2, :]) * 0.5     # d psi / dy

@@ -27,7 +27,7 @@ extern "C" {
 
 typedef enum eqf_status {
     EQF_OK = 0,
-    EQF_EINVAL = 1,   /* bad argument (reference: panics, e.g. obstacle.rs:127) */
+    EQF_EINVAL = 1,   /* bad argument (reference: panics, e.g. obstacle.rs:67) */
     EQF_ENOMEM = 2,   /* device or host allocation failed / budget exceeded */
     EQF_EDEVICE = 3,  /* HIP runtime error */
     EQF_EBUSY = 4,    /* object in use (scene owned by a batch or server) */
@@ -93,8 +93,8 @@ int eqf_scene_get_params(const eqf_scene *s, eqf_params *out);
 /* Changes delta_t, diffusion, viscosity, iteration counts; size and mode are fixed. */
 int eqf_scene_set_params(eqf_scene *s, const eqf_params *p);
 
-/* Rectangle::are_all_points_valid (obstacle.rs:134-147); EQF_EINVAL with
- * "Invalid input for Rectangle" (obstacle.rs:127) when invalid. */
+/* Rectangle::are_all_points_valid (obstacle.rs:74-87); EQF_EINVAL with
+ * "Invalid input for Rectangle" (obstacle.rs:67) when invalid. */
 int eqf_rect_validate(int64_t x0, int64_t y0, int64_t x1, int64_t y1, uint32_t size);
 /* Fluid::fill_obstacle (fluid.rs:610-619): walls on [x0,x1) x [y0,y1), clamped. */
 int eqf_fill_rect(eqf_scene *s, int64_t x0, int64_t y0, int64_t x1, int64_t y1);
@@ -149,9 +149,15 @@ int eqf_scene_set_tuning(eqf_scene *s, int tile, int sweeps_per_block);
  * fused launch per frame runs every scene's whole frame.  Red-black mode (any
  * sizes, one iteration count): every kernel of the frame covers all scenes
  * (band passes over all scenes' tiles, then the grid-wide phases).  Scenes
- * stay owned by the caller; while in a batch they must not be stepped
- * separately, and their mask, parameters and frame sources are fixed
- * (EQF_EBUSY). */
+ * stay owned by the caller.  While in a batch a scene is not stepped alone
+ * and its mask, parameters and frame sources are fixed (eqf_step,
+ * eqf_fill_rect, eqf_set_mask, eqf_scene_set_params, eqf_set_frame_source
+ * and eqf_scene_destroy return EQF_EBUSY).  The other scene calls (adds,
+ * uploads, downloads, clone, render, snapshots, eqf_sync) stay available and
+ * are ordered with the batch: each runs after the batch steps queued before
+ * it, and the next eqf_batch_step runs after it -- the reference's frame loop
+ * (add sources, then step) works on batched scenes.  A scene appears in a
+ * batch at most once (EQF_EINVAL). */
 int eqf_batch_create(eqf_ctx *ctx, eqf_scene *const *scenes, int n, eqf_batch **out);
 int eqf_batch_step(eqf_batch *b, int64_t nframes);
 int eqf_batch_sync(eqf_batch *b);

@@ -54,6 +54,7 @@ def load_library(path):
     L.ora_idx.argtypes = [I64, I64, I64]
     L.ora_idx.restype = I64
     L.ora_step_par_f64.argtypes = [P] * 6 + [I64, ctypes.c_double, ctypes.c_double, ctypes.c_double, I64, I64, P]
+    L.ora_step_par_f32.argtypes = [P] * 6 + [I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, I64, I64, P]
     L.ora_poisson_residual_f64.argtypes = [P, P, I64, P, P]
     L.ora_poisson_residual_f64.restype = ctypes.c_double
     L.ora_render_rgba.argtypes = [P, P, I64, P, P, P, P]
@@ -148,11 +149,17 @@ class State:
             iters if press_iters is None else press_iters, _p(self.mask), order,
             _p(sd), _p(su), _p(sv), nframes)
 
-    def step_par_f64(self, dt, diffusion, viscosity, iters, press_iters=None):
+    def step_par(self, dt, diffusion, viscosity, iters, press_iters=None):
+        """One red-black step (OpenMP), same bits as run(..., order=RB) for
+        one frame without sources; f32 or f64 by the state's dtype."""
         f = self.f
-        assert f["density"].dtype == np.float64
-        lib().ora_step_par_f64(*(_p(f[k]) for k in FIELDS), self.size, dt, diffusion, viscosity,
-                               iters, iters if press_iters is None else press_iters, _p(self.mask))
+        getattr(lib(), "ora_step_par" + _sfx(f["density"]))(
+            *(_p(f[k]) for k in FIELDS), self.size, dt, diffusion, viscosity, iters,
+            iters if press_iters is None else press_iters, _p(self.mask))
+
+    def step_par_f64(self, dt, diffusion, viscosity, iters, press_iters=None):
+        assert self.f["density"].dtype == np.float64
+        self.step_par(dt, diffusion, viscosity, iters, press_iters)
 
 
 def set_bnd(mode, x, size, mask):

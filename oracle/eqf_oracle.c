@@ -22,9 +22,10 @@
  *   ora_run          renderer_helpers.rs:54-59 frame loop
  *   ora_init         fluid.rs:93-110, 527-570, 602-606 (+ Default, 83-89)
  *   ora_fill_rect    fluid.rs:610-619
- *   ora_rect_valid   obstacle.rs:134-147
- *   ora_step_par_f64 parallel (OpenMP) red-black f64 step, identical results
- *                    to ora_step_f64(order=ORA_RB), for large grids.
+ *   ora_rect_valid   obstacle.rs:74-87
+ *   ora_step_par_f32 / ora_step_par_f64
+ *                    parallel (OpenMP) red-black step, identical results to
+ *                    ora_step_f32/_f64(order=ORA_RB), for large grids.
  *   ora_render_rgba  renderer_helpers.rs:143-165 (RenderingListener::render_image
  *                    pixel rule, `f32 as u8` with Rust's saturating semantics)
  *
@@ -135,7 +136,7 @@ void ora_fill_rect(uint8_t *m, uint32_t S, int64_t x0, int64_t y0, int64_t x1, i
         for (int64_t y = y0; y < y1; y++) m[IDX(x, y, (int64_t)S)] = ORA_WALL;
 }
 
-/* `Rectangle::are_all_points_valid` (obstacle.rs:134-147). */
+/* `Rectangle::are_all_points_valid` (obstacle.rs:74-87). */
 int ora_rect_valid(int64_t x0, int64_t y0, int64_t x1, int64_t y1, int64_t size) {
     return x0 != x1 && y0 != y1 && x0 < x1 && y0 < y1 && x0 < size && y0 < size && x1 < size &&
            y1 < size;
@@ -149,133 +150,23 @@ void ora_to_coordinate(int64_t idx, int64_t size, int64_t *x, int64_t *y) {
 
 int64_t ora_idx(int64_t x, int64_t y, int64_t size) { return (int64_t)IDX(x, y, size); }
 
-/* ------------------------------------------- parallel red-black f64 step --
- * Same arithmetic and order as ora_step_f64(..., ORA_RB): every loop that is
- * parallelised here writes disjoint cells and reads only cells that the loop
- * does not write (red cells read black neighbours and vice versa; advect rows
- * are independent; set_boundaries phase 1 writes only fluid cells or ring
- * cells and reads only wall or interior cells).  Passive phase 1 is written
- * per ring row/column ("copy if that column/row holds a NoWall cell"), which
- * is what the cell scan produces.  tests/test_oracle.py checks equality with
- * the sequential oracle. */
+/* ---------------------------------------- parallel red-black f32/f64 step --
+ * ora_step_par_f32 / ora_step_par_f64: body in eqf_oracle_par.h. */
+#define T float
+#define ORA_SFX _f32
+#define ORA_FLOOR floorf
+#include "eqf_oracle_par.h"
+#undef T
+#undef ORA_SFX
+#undef ORA_FLOOR
 
-static void par_set_bnd(int mode, double *x, int64_t S, const uint8_t *m, const uint8_t *colf,
-                        const uint8_t *rowf) {
-    if (mode == ORA_PASSIVE) {
-#pragma omp parallel for schedule(static)
-        for (int64_t i = 1; i < S - 1; i++) {
-            if (colf[i]) {
-                x[i] = x[i + S];
-                x[i + (S - 1) * S] = x[i + (S - 2) * S];
-            }
-            if (rowf[i]) {
-                x[i * S] = x[1 + i * S];
-                x[S - 1 + i * S] = x[S - 2 + i * S];
-            }
-        }
-    } else {
-#pragma omp parallel for schedule(static)
-        for (int64_t j = 1; j < S - 1; j++)
-            for (int64_t i = 1; i < S - 1; i++) ora_bnd_cell_f64(mode, x, S, m, i, j);
-    }
-    x[0] = 0.5 * (x[1] + x[S]);
-    x[(S - 1) * S] = 0.5 * (x[1 + (S - 1) * S] + x[(S - 2) * S]);
-    x[S - 1] = 0.5 * (x[S - 2] + x[S - 1 + S]);
-    x[S - 1 + (S - 1) * S] = 0.5 * (x[S - 2 + (S - 1) * S] + x[S - 1 + (S - 2) * S]);
-}
-
-static void par_lin_solve(int mode, double *x, const double *x0, double a, double c, int64_t S,
-                          int64_t iters, const uint8_t *m, const uint8_t *colf,
-                          const uint8_t *rowf) {
-    double cr = 1.0 / c;
-    for (int64_t k = 0; k < iters; k++) {
-        for (int color = 0; color < 2; color++) {
-#pragma omp parallel for schedule(static)
-            for (int64_t j = 1; j < S - 1; j++)
-                for (int64_t i = 1 + ((j + 1 + color) & 1); i < S - 1; i += 2) {
-                    size_t q = (size_t)(i + j * S);
-                    x[q] = (x0[q] + a * (((x[q + 1] + x[q - 1]) + x[q + S]) + x[q - S])) * cr;
-                }
-        }
-        par_set_bnd(mode, x, S, m, colf, rowf);
-    }
-}
-
-static void par_project(double *u, double *v, double *p, double *div, int64_t S, int64_t iters,
-                        const uint8_t *m, const uint8_t *colf, const uint8_t *rowf) {
-    double sz = (double)S;
-#pragma omp parallel for schedule(static)
-    for (int64_t j = 1; j < S - 1; j++)
-        for (int64_t i = 1; i < S - 1; i++) {
-            size_t q = (size_t)(i + j * S);
-            div[q] = -0.5 * (u[q + 1] - u[q - 1] + v[q + S] - v[q - S]) / sz;
-            p[q] = 0.0;
-        }
-    par_set_bnd(ORA_PASSIVE, div, S, m, colf, rowf);
-    par_set_bnd(ORA_PASSIVE, p, S, m, colf, rowf);
-    par_lin_solve(ORA_PASSIVE, p, div, 1.0, 4.0, S, iters, m, colf, rowf);
-#pragma omp parallel for schedule(static)
-    for (int64_t j = 1; j < S - 1; j++)
-        for (int64_t i = 1; i < S - 1; i++) {
-            size_t q = (size_t)(i + j * S);
-            u[q] -= 0.5 * (p[q + 1] - p[q - 1]) * sz;
-            v[q] -= 0.5 * (p[q + S] - p[q - S]) * sz;
-        }
-    par_set_bnd(ORA_ROW, u, S, m, colf, rowf);
-    par_set_bnd(ORA_COL, v, S, m, colf, rowf);
-}
-
-static void par_advect(int mode, double *d, const double *d0, const double *u, const double *v,
-                       int64_t S, double dt, const uint8_t *m, const uint8_t *colf,
-                       const uint8_t *rowf) {
-    double dt0 = dt * (double)(S - 2);
-    double sf = (double)S;
-#pragma omp parallel for schedule(static)
-    for (int64_t j = 1; j < S - 1; j++) {
-        for (int64_t i = 1; i < S - 1; i++) {
-            double x = (double)i - dt0 * u[i + j * S];
-            double y = (double)j - dt0 * v[i + j * S];
-            x = ora_clamp_f64(x, 0.5, sf - 1.0);
-            y = ora_clamp_f64(y, 0.5, sf - 1.0);
-            double i0 = floor(x), i1 = i0 + 1.0, j0 = floor(y), j1 = j0 + 1.0;
-            double s1 = x - i0, s0 = 1.0 - s1, t1 = y - j0, t0 = 1.0 - t1;
-            int64_t i0i = sat_u32(i0), i1i = sat_u32(i1), j0i = sat_u32(j0), j1i = sat_u32(j1);
-            if (i1 >= sf || j1 >= sf) {
-                d[i + j * S] = d[i - 1 + j * S];
-                break;
-            }
-            d[i + j * S] = s0 * (t0 * d0[IDX(i0i, j0i, S)] + t1 * d0[IDX(i0i, j1i, S)]) +
-                           s1 * (t0 * d0[IDX(i1i, j0i, S)] + t1 * d0[IDX(i1i, j1i, S)]);
-        }
-    }
-    par_set_bnd(mode, d, S, m, colf, rowf);
-}
-
-void ora_step_par_f64(double *D, double *Sc, double *U, double *V, double *U0, double *V0,
-                      int64_t S, double dt, double diffusion, double viscosity,
-                      int64_t diff_iters, int64_t press_iters, const uint8_t *m) {
-    uint8_t *colf = (uint8_t *)calloc((size_t)S, 1);
-    uint8_t *rowf = (uint8_t *)calloc((size_t)S, 1);
-    for (int64_t j = 0; j < S; j++)
-        for (int64_t i = 0; i < S; i++)
-            if (m[i + j * S] != ORA_WALL) {
-                colf[i] = 1;
-                rowf[j] = 1;
-            }
-    double sf = (double)(S - 2);
-    double av = dt * viscosity * sf * sf, ad = dt * diffusion * sf * sf;
-    par_lin_solve(ORA_ROW, U0, U, av, 1.0 + 4.0 * av, S, diff_iters, m, colf, rowf);
-    par_lin_solve(ORA_COL, V0, V, av, 1.0 + 4.0 * av, S, diff_iters, m, colf, rowf);
-    par_project(U0, V0, U, V, S, press_iters, m, colf, rowf);
-    par_advect(ORA_ROW, U, U0, U0, V0, S, dt, m, colf, rowf);
-    par_advect(ORA_COL, V, V0, U0, V0, S, dt, m, colf, rowf);
-    par_project(U, V, U0, V0, S, press_iters, m, colf, rowf);
-    par_lin_solve(ORA_PASSIVE, Sc, D, ad, 1.0 + 4.0 * ad, S, diff_iters, m, colf, rowf);
-    par_advect(ORA_PASSIVE, D, Sc, U, V, S, dt, m, colf, rowf);
-    memcpy(Sc, D, (size_t)(S * S) * sizeof(double));
-    free(colf);
-    free(rowf);
-}
+#define T double
+#define ORA_SFX _f64
+#define ORA_FLOOR floor
+#include "eqf_oracle_par.h"
+#undef T
+#undef ORA_SFX
+#undef ORA_FLOOR
 
 /* Residual of the pressure Poisson system solved by project's lin_solve:
  * r = div + (sum of the 4 neighbours of p) - 4p over interior cells.

@@ -114,3 +114,17 @@ def test_render_and_snapshot_reject_null_arguments():
     assert L.eqf_snapshot_acquire(None, None, None, None) == _abi.EQF_EINVAL
     assert L.eqf_snapshot_release(None) == _abi.EQF_EINVAL
     assert L.eqf_snapshot_destroy(None) == _abi.EQF_OK
+
+
+def test_batch_rejects_a_scene_listed_twice():
+    """A scene appearing twice in one batch would be advanced twice per frame
+    by concurrent workgroups; eqf_batch_create refuses it before touching the
+    device or the scenes (the handles below are never dereferenced)."""
+    L = _abi.lib()
+    fake_ctx = ctypes.create_string_buffer(64)
+    fake = ctypes.create_string_buffer(64)
+    other = ctypes.create_string_buffer(64)
+    arr = (ctypes.c_void_p * 3)(ctypes.addressof(fake), ctypes.addressof(other), ctypes.addressof(fake))
+    h = ctypes.c_void_p()
+    assert L.eqf_batch_create(ctypes.addressof(fake_ctx), arr, 3, ctypes.byref(h)) == _abi.EQF_EINVAL
+    assert "more than once" in _abi.last_error()

@@ -300,7 +300,7 @@ def _build_mutant(tmp, name, edits):
     src_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
     d = os.path.join(tmp, name)
     os.makedirs(d)
-    for f in ("eqf_oracle.c", IMPL):
+    for f in ("eqf_oracle.c", IMPL, "eqf_oracle_par.h"):
         shutil.copy(os.path.join(src_dir, f), d)
     path = os.path.join(d, IMPL)
     text = open(path).read()
