@@ -64,6 +64,23 @@ def build(verbose=False):
     return OUT
 
 
+BWPROBE_OUT = os.path.join(PKG_DIR, "libeqf_bwprobe.so")
+
+
+def build_bwprobe(verbose=False):
+    """libeqf_bwprobe.so: the streaming kernels that measure the practical
+    HBM bandwidth ceiling (loaded by bench.py only)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc] + FLAGS + [os.path.join(CSRC, "eqf_bwprobe.hip"), "-o", BWPROBE_OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(BWPROBE_OUT + ".tmp", BWPROBE_OUT)
+    return BWPROBE_OUT
+
+
 def build_cpp_tests(verbose=False):
     """g++ build of tests/cpp/test_fluid_api (host code only) against
     libeqfluid.so and the oracle library."""
