@@ -149,6 +149,7 @@ def lib():
             "eqf_server_destroy": (I, [P]),
             "eqf_colors_default": (None, [ctypes.POINTER(Colors)]),
             "eqf_render_rgba": (I, [P, ctypes.POINTER(Colors), P]),
+            "eqf_render_rgba_noalloc": (I, [P, ctypes.POINTER(Colors), P]),
             "eqf_snapshot_create": (I, [P, I, I, ctypes.POINTER(Colors), PP]),
             "eqf_snapshot_capture": (I, [P, I64]),
             "eqf_snapshot_acquire": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(ctypes.POINTER(F)),
@@ -300,6 +301,16 @@ class Scene:
         """(size*size, 4) uint8 image of the current state (eqf_render_rgba)."""
         out = np.empty((self.size * self.size, 4), np.uint8)
         check(lib().eqf_render_rgba(self.h, None if colors is None else ctypes.byref(colors), ptr(out)))
+        return out
+
+    def render_rgba_noalloc(self, colors=None, out=None):
+        """Same image through eqf_render_rgba_noalloc (no device allocation;
+        fixed per-GPU scratch); `out` may be a preallocated (size*size, 4)
+        uint8 array."""
+        if out is None:
+            out = np.empty((self.size * self.size, 4), np.uint8)
+        assert out.dtype == np.uint8 and out.size == 4 * self.size * self.size and out.flags["C_CONTIGUOUS"]
+        check(lib().eqf_render_rgba_noalloc(self.h, None if colors is None else ctypes.byref(colors), ptr(out)))
         return out
 
 

@@ -61,6 +61,9 @@ int eqf_scene_waves(int ns, int P);
 hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int waves, int lds_floats, int nframes,
                                hipStream_t st);
 int eqf_internal_error(int code, const std::string& msg);
+// Device address of eqf_render_rgba_noalloc's fixed scratch on `device`
+// (eqf_render.hip), looked up once per device at context creation.
+int eqf_internal_scratch_init(int device);
 // Orders work about to be queued on the scene's stream after the last step of
 // the scene's batch (no-op outside a batch).
 int eqf_internal_scene_enter(eqf_scene* s);

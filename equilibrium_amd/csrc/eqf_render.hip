@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -240,5 +241,87 @@ extern "C" int eqf_snapshot_destroy(eqf_snapshot* sn) {
   (void)hipSetDevice(sn->device);
   (void)hipStreamSynchronize(sn->copy);
   snapshot_free(sn);
+  return EQF_OK;
+}
+
+// ------------------------------------------------- allocation-free render --
+// eqf_render_rgba_noalloc stages the row-major density and the RGBA image in
+// `scratch`, a fixed device array of EQF_NOALLOC_SCRATCH_BYTES that every GPU
+// the library runs on holds until the process ends, instead of the device
+// buffer eqf_render_rgba allocates and frees per call.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current (eqf_ctx_create calls eqf_internal_scratch_init), and only that
+// address is used below.  The staging is fully rewritten by each call, so
+// nothing in it must start at zero.
+static __device__ __align__(256) unsigned char scratch[EQF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                  // guards g_scratch_ptr
+void* g_scratch_ptr[kMaxDevices] = {};    // device address of `scratch`, per device
+std::mutex g_scratch_use[kMaxDevices];    // one call at a time per device
+}  // namespace
+
+int eqf_internal_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return eqf_internal_error(EQF_EINVAL, "device index too large for the render scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return EQF_OK;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+  void* p = nullptr;
+  if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return eqf_internal_error(EQF_EDEVICE, std::string("render scratch address: ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return EQF_OK;
+}
+
+extern "C" int eqf_render_rgba_noalloc(eqf_scene* s, const eqf_colors* c, uint8_t* rgba_host) {
+  if (!s || !rgba_host) return eqf_internal_error(EQF_EINVAL, "bad arguments");
+  const size_t n = (size_t)s->S * s->S;
+  const size_t need = sizeof(float) * rgba_offset_floats(n) + 4 * n;
+  if (need > (size_t)EQF_NOALLOC_SCRATCH_BYTES)
+    return eqf_internal_error(EQF_EINVAL, "scene too large for eqf_render_rgba_noalloc (size <= " +
+                                              std::to_string(EQF_NOALLOC_RENDER_MAX_SIZE) +
+                                              "); use eqf_render_rgba");
+  // The device mask is built by the first step after a mask change (it is
+  // allocated then); this entry point only reads it.
+  if (s->bnd_dirty || !s->d_mask)
+    return eqf_internal_error(EQF_EINVAL, "the scene's mask changed since its last step (or it was never "
+                                          "stepped): step it, or call eqf_render_rgba, first");
+  const int dev = s->ctx->device;
+  if (dev < 0 || dev >= kMaxDevices) return eqf_internal_error(EQF_EINVAL, "device index too large");
+  hipError_t e = hipSetDevice(dev);
+  if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, hipGetErrorString(e));
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[dev];
+  }
+  if (!base) {  // a context created by an older copy of the library state
+    int rc = eqf_internal_scratch_init(dev);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[dev];
+  }
+  eqf_colors def;
+  if (!c) {
+    eqf_colors_default(&def);
+    c = &def;
+  }
+  std::lock_guard<std::mutex> use(g_scratch_use[dev]);
+  int rc = eqf_internal_scene_enter(s);
+  if (rc) return rc;
+  float* d_dens = (float*)base;
+  uint8_t* d_rgba = (uint8_t*)(d_dens + rgba_offset_floats(n));
+  rc = eqf_dev_to_rm_async(s, s->d_fld + (int64_t)F_D * s->fstride, d_dens);
+  if (!rc) rc = launch_render(s, d_dens, c, d_rgba);
+  if (rc) return rc;
+  e = hipMemcpyAsync(rgba_host, d_rgba, 4 * n, hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);  // the next call may reuse the scratch
+  if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, std::string("render copy: ") + hipGetErrorString(e));
   return EQF_OK;
 }

@@ -182,6 +182,7 @@ extern "C" int eqf_ctx_create(int device, eqf_ctx** out) {
   HIPCHK(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(EQF_EINVAL, "no such device");
   HIPCHK(hipSetDevice(device));
+  if (int rc = eqf_internal_scratch_init(device)) return rc;  // with the device current
   auto* c = new (std::nothrow) eqf_ctx;
   if (!c) return fail(EQF_ENOMEM, "host allocation failed");
   c->device = device;

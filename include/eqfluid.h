@@ -260,6 +260,23 @@ void eqf_colors_default(eqf_colors *c);
  * colours = eqf_colors_default.  Synchronous. */
 int eqf_render_rgba(eqf_scene *s, const eqf_colors *c, uint8_t *rgba);
 
+/* Allocation-free variant of eqf_render_rgba (no counterpart in the
+ * reference; for callers that cannot allocate device memory: render loops
+ * that must not call hipMalloc/hipFree, which also synchronise the device,
+ * and harnesses that forbid it).  It calls no hipMalloc, hipFree or other
+ * allocating routine: the density and image are staged in a fixed device
+ * array of EQF_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it
+ * runs on until the process ends.  Limits: size <= EQF_NOALLOC_RENDER_MAX_SIZE
+ * (EQF_EINVAL beyond), and the scene's device mask must be current -- after
+ * eqf_fill_rect / eqf_set_mask, step the scene (or call eqf_render_rgba) once
+ * before using this call (EQF_EINVAL otherwise).  Calls on one GPU share that
+ * array; the library serialises them with a lock (each call is synchronous),
+ * so concurrent calls from several threads wait for each other rather than
+ * race.  Same output as eqf_render_rgba. */
+#define EQF_NOALLOC_SCRATCH_BYTES (36u << 20)
+#define EQF_NOALLOC_RENDER_MAX_SIZE 2172
+int eqf_render_rgba_noalloc(eqf_scene *s, const eqf_colors *c, uint8_t *rgba);
+
 /* Per-frame snapshots: CurrentSimulation::simulate sends the renderer a
  * clone of the Fluid after every step (renderer_helpers.rs:59-65).  A ring of
  * `depth` (1..8) pinned host slots; a capture is queued behind the scene's
