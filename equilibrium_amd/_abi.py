@@ -146,6 +146,7 @@ def lib():
             "eqf_server_release": (I, [P, ctypes.c_uint64]),
             "eqf_server_wait_idle": (I, [P, ctypes.c_double]),
             "eqf_server_stats_get": (I, [P, ctypes.POINTER(ServerStats)]),
+            "eqf_server_stats_sizes": (I, [P, U32, U32, ctypes.POINTER(ServerStats)]),
             "eqf_server_destroy": (I, [P]),
             "eqf_colors_default": (None, [ctypes.POINTER(Colors)]),
             "eqf_render_rgba": (I, [P, ctypes.POINTER(Colors), P]),
@@ -455,9 +456,15 @@ class Server:
     def wait_idle(self, timeout_ms=-1.0):
         check(lib().eqf_server_wait_idle(self.h, timeout_ms))
 
-    def stats(self):
+    def stats(self, min_size=None, max_size=None):
+        """Server statistics; with min_size / max_size, latencies and counts
+        of the requests of those sizes only (ring-inclusive sizes)."""
         st = ServerStats()
-        check(lib().eqf_server_stats_get(self.h, ctypes.byref(st)))
+        if min_size is None and max_size is None:
+            check(lib().eqf_server_stats_get(self.h, ctypes.byref(st)))
+        else:
+            check(lib().eqf_server_stats_sizes(self.h, 0 if min_size is None else min_size,
+                                               0xFFFFFFFF if max_size is None else max_size, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
     def close(self):

@@ -47,7 +47,17 @@ struct eqf_scene {
   bool ops_pending = false;
   bool owns_stream = true;
   void* rb = nullptr;       // red-black workspace (eqf_redblack.hip)
+  // Scenes of a server allocate from the server's memory pool, ordered on
+  // pool_stream (hipMallocFromPoolAsync / hipFreeAsync): freeing a retired
+  // scene then waits for nothing else on the device, where hipFree would wait
+  // for every stream.  Other scenes use hipMalloc / hipFree.
+  hipMemPool_t pool = nullptr;
+  hipStream_t pool_stream = nullptr;
 };
+
+// Device memory of a scene (see eqf_scene::pool).
+hipError_t eqf_scene_malloc(eqf_scene* s, void** p, size_t bytes);
+void eqf_scene_mfree(eqf_scene* s, void* p);
 
 // Runtime services shared with the server (eqf_runtime.hip).
 int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
@@ -68,8 +78,10 @@ int eqf_internal_scratch_init(int device);
 // the scene's batch (no-op outside a batch).
 int eqf_internal_scene_enter(eqf_scene* s);
 int eqf_internal_check_params(const eqf_params* p);
-// eqf_scene_create on a caller-owned stream (the server's admission stream).
-int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, eqf_scene** out);
+// eqf_scene_create on a caller-owned stream (the server's admission stream),
+// allocating from `pool` in that stream's order when `pool` is set.
+int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, hipMemPool_t pool,
+                              eqf_scene** out);
 
 // Device-side setup (eqf_setup.hip).
 int eqf_dev_init_fields(eqf_scene* s, int times);
@@ -83,7 +95,7 @@ void eqf_dev_trim(eqf_scene* s);
 namespace eqf {
 // Red-black mode (eqf_redblack.hip).
 void* rb_create(eqf_scene* s, std::string& err);
-void rb_destroy(void* rb);
+void rb_destroy(eqf_scene* s);
 int rb_step(eqf_scene* s, int64_t nframes, std::string& err);
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps);
 // A batch of red-black scenes: every frame kernel covers all of them.

@@ -951,11 +951,13 @@ static int group_step(RbGroup& g, hipStream_t st, int64_t nframes, std::string& 
 void* rb_create(eqf_scene* s, std::string& err) {
   auto* w = new RBWork;
   const size_t n = (size_t)s->S * s->S;
-  hipError_t e = hipMalloc(&w->tmp, sizeof(float) * n);
-  if (e == hipSuccess) e = hipMalloc(&w->trig, sizeof(int) * (size_t)s->S);
+  hipError_t e = eqf_scene_malloc(s, (void**)&w->tmp, sizeof(float) * n);
+  if (e == hipSuccess) e = eqf_scene_malloc(s, (void**)&w->trig, sizeof(int) * (size_t)s->S);
   if (e != hipSuccess) {
     err = std::string("red-black workspace: ") + hipGetErrorString(e);
-    rb_destroy(w);
+    eqf_scene_mfree(s, w->tmp);
+    eqf_scene_mfree(s, w->trig);
+    delete w;
     return nullptr;
   }
   w->one.scenes = {s};
@@ -964,13 +966,14 @@ void* rb_create(eqf_scene* s, std::string& err) {
   return w;
 }
 
-void rb_destroy(void* p) {
-  auto* w = static_cast<RBWork*>(p);
+void rb_destroy(eqf_scene* s) {
+  auto* w = static_cast<RBWork*>(s->rb);
   if (!w) return;
   group_release(w->one);
-  (void)hipFree(w->tmp);
-  (void)hipFree(w->trig);
+  eqf_scene_mfree(s, w->tmp);
+  eqf_scene_mfree(s, w->trig);
   delete w;
+  s->rb = nullptr;
 }
 
 static bool tiling_ok(int tile, int tsweeps) {

@@ -195,20 +195,32 @@ extern "C" int eqf_ctx_destroy(eqf_ctx* ctx) {
   return EQF_OK;
 }
 
+hipError_t eqf_scene_malloc(eqf_scene* s, void** p, size_t bytes) {
+  if (s->pool) return hipMallocFromPoolAsync(p, bytes, s->pool, s->pool_stream);
+  return hipMalloc(p, bytes);
+}
+
+void eqf_scene_mfree(eqf_scene* s, void* p) {
+  if (!p) return;
+  if (s->pool) (void)hipFreeAsync(p, s->pool_stream);
+  else (void)hipFree(p);
+}
+
 static void scene_free(eqf_scene* s) {
   if (!s) return;
-  if (s->rb) rb_destroy(s->rb);
+  if (s->rb) rb_destroy(s);
   if (s->ev_ops) (void)hipEventDestroy(s->ev_ops);
-  (void)hipFree(s->d_main);
-  (void)hipFree(s->d_src);
-  (void)hipFree(s->d_bnd);
-  (void)hipFree(s->d_stage);
-  (void)hipFree(s->d_mask);
+  eqf_scene_mfree(s, s->d_main);
+  eqf_scene_mfree(s, s->d_src);
+  eqf_scene_mfree(s, s->d_bnd);
+  eqf_scene_mfree(s, s->d_stage);
+  eqf_scene_mfree(s, s->d_mask);
   if (s->stream && s->owns_stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
-static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipStream_t stream = nullptr) {
+static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipStream_t stream = nullptr,
+                       hipMemPool_t pool = nullptr) {
   int rc = check_params(p);
   if (rc) return rc;
   HIPCHK(hipSetDevice(ctx->device));
@@ -228,6 +240,10 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
   if (stream) {
     s->stream = stream;
     s->owns_stream = false;
+    if (pool) {
+      s->pool = pool;
+      s->pool_stream = stream;
+    }
   } else {
     e = hipStreamCreate(&s->stream);
   }
@@ -240,7 +256,7 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
   const size_t b_fld = up(sizeof(float) * NFIELDS * (size_t)s->fstride);
   const size_t b_code = up(ncode), b_rowf = up((size_t)s->S);
   const size_t b_desc = up(sizeof(SceneDesc));
-  e = hipMalloc(&s->d_main, b_fld + b_code + b_rowf + b_desc);
+  e = eqf_scene_malloc(s, &s->d_main, b_fld + b_code + b_rowf + b_desc);
   if (e == hipSuccess) {
     char* base = (char*)s->d_main;
     s->d_fld = (float*)base;
@@ -259,13 +275,14 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
 }
 
 extern "C" int eqf_scene_create(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out) {
-  return eqf_internal_scene_create(ctx, p, nullptr, out);
+  return eqf_internal_scene_create(ctx, p, nullptr, nullptr, out);
 }
 
-int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, eqf_scene** out) {
+int eqf_internal_scene_create(eqf_ctx* ctx, const eqf_params* p, hipStream_t stream, hipMemPool_t pool,
+                              eqf_scene** out) {
   if (!ctx || !out) return fail(EQF_EINVAL, "ctx/out is NULL");
   eqf_scene* s = nullptr;
-  int rc = scene_alloc(ctx, p, &s, stream);
+  int rc = scene_alloc(ctx, p, &s, stream, pool);
   if (rc) return rc;
   if (s->S >= 22) {
     // No index clamping in init_density at this size: write the state on the
@@ -312,7 +329,7 @@ extern "C" int eqf_scene_clone(const eqf_scene* src, eqf_scene** out) {
                        s->stream);
   s->src_mask = src->src_mask;
   if (e == hipSuccess && src->d_src) {
-    e = hipMalloc(&s->d_src, sizeof(float) * 3 * s->fstride);
+    e = eqf_scene_malloc(s, (void**)&s->d_src, sizeof(float) * 3 * s->fstride);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s->d_src, src->d_src, sizeof(float) * 3 * s->fstride, hipMemcpyDeviceToDevice, s->stream);
   }
@@ -486,7 +503,7 @@ extern "C" int eqf_set_frame_source(eqf_scene* s, int field, const float* amount
   const int slot = field == EQF_DENSITY ? 0 : field == EQF_VX ? 1 : 2;
   if (!s->d_src) {
     if (!amounts) return EQF_OK;
-    HIPCHK(hipMalloc(&s->d_src, sizeof(float) * 3 * s->fstride));
+    HIPCHK(eqf_scene_malloc(s, (void**)&s->d_src, sizeof(float) * 3 * s->fstride));
     HIPCHK(hipMemsetAsync(s->d_src, 0, sizeof(float) * 3 * s->fstride, s->stream));
   }
   float* dst = s->d_src + (int64_t)slot * s->fstride;

@@ -1,17 +1,20 @@
 // eqf_server.hip -- continuous batching of scene requests on one GPU.
 //
-// A server thread owns every admitted scene.  Each tick it launches ONE fused
-// frame kernel (eqf_frames_cyclic, grid = resident wavefront-mode scenes) that
-// advances every such scene by one frame (fluid.rs:437-524); red-black scenes
-// (any size, for grids past the wavefront's 1026 limit) advance beside it by
-// one frame of their launch group (every kernel covers all of them).  While the
-// tick runs,
-// the thread admits queued requests whose device memory fits the budget:
-// it allocates them, writes Fluid::new's state, marks obstacles and uploads
-// sources on each scene's own stream, so admission overlaps the running tick.
-// Admitted scenes join at the next tick; finished scenes are read back and
-// freed at tick boundaries.  Every scene runs its own workgroup with its own
-// data, so a scene's result does not depend on which others share its ticks.
+// A server thread owns every admitted scene.  Resident scenes are split into
+// lanes that tick independently, each on its own stream: small wavefront-mode
+// scenes (at most 4 strips of 64 rows, N <= 256), large wavefront-mode scenes
+// (5-16 strips), and red-black scenes (any size).  A lane's tick advances its
+// scenes by one frame (fluid.rs:437-524): one fused launch of
+// eqf_frames_cyclic over the lane's scenes, sized by the lane's own largest
+// scene, or one frame graph per red-black launch group (one group per pair of
+// iteration counts).  When a lane's tick ends, the thread retires the lane's
+// finished scenes, adds the ones admitted to it meanwhile and launches its
+// next tick, so a small scene's frame takes its own lane's time rather than
+// the largest resident scene's.  Between those events the thread admits
+// queued requests while their device memory fits the budget (allocation,
+// Fluid::new's state, obstacles, sources) on a fourth stream, overlapping the
+// running ticks.  Every scene runs its own workgroups with its own data, so a
+// scene's result does not depend on which others share its ticks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,7 +52,8 @@ struct Req {
   int64_t frames_done = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
   eqf_scene* scene = nullptr;
-  bool rb = false;  // red-black mode: own frame graph on the red-black stream
+  bool rb = false;  // red-black mode: advanced by its launch group on the red-black lane
+  int lane = 0;
   uint64_t bytes = 0;
   int64_t cells = 0;
   std::vector<float> out_d, out_u, out_v;
@@ -57,38 +61,66 @@ struct Req {
 
 }  // namespace
 
+namespace {
+// Lanes of resident scenes (see the file comment).  With the admission stream
+// that is four streams, the number of hardware queues HIP gives a process by
+// default: streams sharing a queue would run their ticks one after the other.
+enum { LANE_SMALL = 0, LANE_LARGE = 1, LANE_RB = 2, NLANES = 3 };
+constexpr int kSmallMaxStrips = 4;
+
+struct Lane {
+  hipStream_t st = nullptr;
+  hipEvent_t ev = nullptr;          // recorded after each tick
+  std::vector<Req*> members;        // resident scenes of the lane
+  std::vector<Req*> ticking;        // scenes of the tick in flight
+  bool busy = false;
+  double t_launch = 0;
+  // wavefront lanes: the descriptors of the scenes last launched
+  SceneDesc* d_descs = nullptr;
+  int desc_cap = 0;
+  std::vector<uint64_t> last_uids;  // scene uids (never reused) whose descriptors d_descs holds
+  std::vector<SceneDesc> descs;
+};
+
+int lane_of(const eqf_scene* s) {
+  if (s->layout == L_ROWMAJOR) return LANE_RB;
+  return s->ns <= kSmallMaxStrips ? LANE_SMALL : LANE_LARGE;
+}
+}  // namespace
+
 struct eqf_server {
   eqf_ctx* ctx = nullptr;
   eqf_server_config cfg{};
   std::mutex mu;
-  std::condition_variable cv;       // wakes the server thread
+  std::condition_variable cv_tick;  // wakes the tick thread
+  std::condition_variable cv_work;  // wakes the admission thread
   std::condition_variable cv_idle;  // wakes eqf_server_wait_idle
   std::deque<Req*> queue;
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
-  std::vector<Req*> active;
+  std::vector<Req*> active;         // admitted and not yet retired
+  std::vector<Req*> incoming[NLANES];  // admitted, joining their lane at its next tick
+  std::vector<Req*> retiring;       // left their lane, to be read back and freed
   uint64_t next_ticket = 1;
   uint64_t used_bytes = 0;
   uint64_t peak_bytes = 0;
   bool stop = false;
-  std::thread th;
-  hipStream_t stream = nullptr;   // ticks
-  hipStream_t astream = nullptr;  // admissions (overlap the running tick)
-  // red-black scenes' frames, beside the fused launch; two streams so that
-  // the frames of two scenes overlap (small kernels, partly filled grids)
-  static constexpr int kRbStreams = 2;
-  hipStream_t rstream[kRbStreams] = {nullptr, nullptr};
-  int rb_next = 0;
+  std::thread th_tick, th_work;
+  hipStream_t astream = nullptr;  // admissions (overlap the running ticks)
+  // the resident scenes' device memory, allocated and freed in astream's order
+  hipMemPool_t pool = nullptr;
+  Lane lanes[NLANES];
   // launch groups of the resident red-black scenes, one per (diffusion,
-  // pressure) iteration counts: every frame kernel covers all scenes of a group
+  // pressure) iteration counts, on the red-black lane's stream: every frame
+  // kernel covers all scenes of a group
   std::map<std::pair<int, int>, void*> rb_groups;
-  SceneDesc* d_descs = nullptr;
-  int desc_cap = 0;
   Clock::time_point t0;
   // statistics
   int64_t ticks = 0, scene_frames = 0, max_resident = 0;
   double cell_updates = 0, busy_ms = 0;
-  std::vector<std::pair<double, int>> tick_lat;  // (ms, resident scenes)
-  std::vector<double> queue_lat;
+  int busy_lanes = 0;
+  double busy_since = 0;
+  std::vector<std::pair<uint32_t, float>> frame_lat;   // (size, tick ms) per scene-frame
+  std::vector<std::pair<uint32_t, double>> queue_lat;  // (size, submit -> admit ms) per request
   std::string thread_error;
 
   double now_ms() const {
@@ -112,7 +144,7 @@ static uint64_t estimate_bytes(const eqf_params& p) {
 
 static int admit(eqf_server* sv, Req* r, std::string& err) {
   eqf_scene* s = nullptr;
-  int rc = eqf_internal_scene_create(sv->ctx, &r->p, sv->astream, &s);
+  int rc = eqf_internal_scene_create(sv->ctx, &r->p, sv->astream, sv->pool, &s);
   for (size_t k = 0; !rc && k + 4 <= r->rects.size(); k += 4)
     rc = eqf_fill_rect(s, r->rects[k], r->rects[k + 1], r->rects[k + 2], r->rects[k + 3]);
   auto in = [](const std::vector<float>& v, const float* b) { return v.empty() ? b : v.data(); };
@@ -131,10 +163,10 @@ static int admit(eqf_server* sv, Req* r, std::string& err) {
     return rc;
   }
   r->rb = s->layout == L_ROWMAJOR;
-  if (r->rb) {  // its frames (and graph) live on a red-black stream
-    s->stream = sv->rstream[sv->rb_next];
-    sv->rb_next = (sv->rb_next + 1) % eqf_server::kRbStreams;
-  }
+  r->lane = lane_of(s);
+  // the scene keeps the admission stream for its own operations (its
+  // read-back at retirement, after its lane's last tick has ended); its
+  // frames run on its lane's stream
   r->scene = s;
   r->cells = (int64_t)(r->p.size - 2) * (r->p.size - 2);
   // release the host copies of the inputs (borrowed ones are not read again)
@@ -163,226 +195,243 @@ static void finish(eqf_server* sv, Req* r) {
   r->scene = nullptr;
 }
 
-static void server_loop(eqf_server* sv) {
+// A lane's tick has ended (called with sv->mu held).
+static void lane_tick_done(eqf_server* sv, Lane& L, double now) {
+  const float ms = (float)(now - L.t_launch);
+  sv->ticks++;
+  for (Req* r : L.ticking) {
+    r->frames_done++;
+    sv->scene_frames++;
+    sv->cell_updates += (double)r->cells;
+    sv->frame_lat.emplace_back(r->p.size, ms);
+  }
+  L.ticking.clear();
+  L.busy = false;
+  if (--sv->busy_lanes == 0) sv->busy_ms += now - sv->busy_since;
+}
+
+// Launches one frame of every scene of an idle lane.  Returns an error
+// message, or "" on success.
+static std::string lane_launch(eqf_server* sv, int li) {
+  Lane& L = sv->lanes[li];
+  L.ticking = L.members;
+  hipError_t e = hipSuccess;
+  std::string err;
+  if (li != LANE_RB) {
+    auto& wf = L.ticking;
+    std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
+    int maxns = 1, lds = 0, waves = 1;
+    for (Req* r : wf) {
+      maxns = std::max(maxns, r->scene->ns);
+      lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
+      waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
+    }
+    std::vector<uint64_t> uids(wf.size());
+    for (size_t i = 0; i < wf.size(); ++i) uids[i] = wf[i]->scene->uid;
+    if (uids != L.last_uids) {
+      L.descs.resize(wf.size());
+      for (size_t i = 0; i < wf.size(); ++i) {
+        L.descs[i] = wf[i]->scene->desc;
+        L.descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
+      }
+      L.last_uids.clear();
+      if ((int)wf.size() > L.desc_cap) {
+        // the lane is idle: nothing reads the old array any more
+        const int cap = std::max<int>((int)wf.size(), 2 * L.desc_cap);
+        (void)hipFree(L.d_descs);
+        L.d_descs = nullptr;
+        L.desc_cap = 0;
+        e = hipMalloc(&L.d_descs, sizeof(SceneDesc) * cap);
+        if (e == hipSuccess) L.desc_cap = cap;
+      }
+      // pageable source: the copy has read it when the call returns
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(L.d_descs, L.descs.data(), sizeof(SceneDesc) * wf.size(), hipMemcpyHostToDevice, L.st);
+      if (e == hipSuccess) L.last_uids = uids;
+    }
+    if (e == hipSuccess) e = eqf_internal_launch(L.d_descs, (int)wf.size(), waves, lds, 1, L.st);
+    if (e != hipSuccess) err = hipGetErrorString(e);
+  } else {
+    std::map<std::pair<int, int>, std::vector<eqf_scene*>> by_iters;
+    std::stable_sort(L.ticking.begin(), L.ticking.end(),
+                     [](const Req* a, const Req* b) { return a->cells > b->cells; });
+    for (Req* r : L.ticking) by_iters[{r->scene->desc.diff_iters, r->scene->desc.press_iters}].push_back(r->scene);
+    for (auto& [key, members] : by_iters) {
+      void*& g = sv->rb_groups[key];
+      if (!g) g = rb_batch_create(members, L.st, err);
+      if (!g) break;
+      rb_batch_set_scenes(g, members);
+      if (rb_batch_step(g, 1, err) != EQF_OK) break;
+    }
+  }
+  if (err.empty()) {
+    e = hipEventRecord(L.ev, L.st);
+    if (e != hipSuccess) err = hipGetErrorString(e);
+  }
+  if (!err.empty()) {
+    L.ticking.clear();
+    return err;
+  }
+  std::lock_guard<std::mutex> lk(sv->mu);
+  L.busy = true;
+  L.t_launch = sv->now_ms();
+  if (sv->busy_lanes++ == 0) sv->busy_since = L.t_launch;
+  return "";
+}
+
+// Admits the queue's head if its memory fits; true if one was admitted (or
+// dropped).
+static bool admit_one(eqf_server* sv) {
+  Req* r = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    while (!sv->queue.empty() && sv->queue.front()->cancel) {
+      Req* h = sv->queue.front();
+      sv->queue.pop_front();
+      h->state = EQF_REQ_CANCELLED;
+      if (h->forget) sv->reqs.erase(h->ticket);
+    }
+    if (sv->queue.empty()) return false;
+    Req* h = sv->queue.front();
+    if (sv->cfg.max_active > 0 && sv->active.size() >= (size_t)sv->cfg.max_active) return false;
+    if (sv->used_bytes + h->bytes > sv->cfg.hbm_budget_bytes) return false;
+    sv->queue.pop_front();
+    sv->used_bytes += h->bytes;
+    sv->peak_bytes = std::max(sv->peak_bytes, sv->used_bytes);
+    r = h;
+  }
+  std::string err;
+  if (admit(sv, r, err) != EQF_OK) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->used_bytes -= r->bytes;
+    r->state = EQF_REQ_FAILED;
+    sv->thread_error = err;
+    return true;
+  }
+  r->t_admit = sv->now_ms();
+  if (r->nframes <= 0) {  // nothing to run
+    finish(sv, r);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->used_bytes -= r->bytes;
+    r->state = r->cancel ? EQF_REQ_CANCELLED : EQF_REQ_DONE;
+    r->t_done = sv->now_ms();
+    if (r->forget) sv->reqs.erase(r->ticket);
+    return true;
+  }
+  std::lock_guard<std::mutex> lk(sv->mu);
+  r->state = EQF_REQ_ACTIVE;
+  sv->active.push_back(r);
+  sv->incoming[r->lane].push_back(r);
+  sv->queue_lat.emplace_back(r->p.size, r->t_admit - r->t_submit);
+  sv->max_resident = std::max<int64_t>(sv->max_resident, (int64_t)sv->active.size());
+  sv->cv_tick.notify_all();
+  return true;
+}
+
+// Tick thread: detects ended ticks, moves an idle lane's finished or
+// cancelled scenes to the retiring list, adds the scenes admitted to it and
+// launches its next tick.  It does no host copies, so a lane restarts within
+// about 100 us of its tick's end whatever the admissions are doing.
+static void tick_loop(eqf_server* sv) {
   (void)hipSetDevice(sv->ctx->device);
-  std::vector<SceneDesc> descs;
-  std::vector<uint64_t> last_uids;  // scenes (by uid: never reused) whose descriptors d_descs holds
   for (;;) {
-    std::vector<Req*> act;
-    {
-      std::unique_lock<std::mutex> lk(sv->mu);
-      sv->cv.wait(lk, [sv] { return sv->stop || !sv->queue.empty() || !sv->active.empty(); });
-      if (sv->stop) break;
-      // drop cancelled resident scenes
-      for (auto it = sv->active.begin(); it != sv->active.end();) {
-        Req* r = *it;
-        if (r->cancel) {
-          lk.unlock();
-          finish(sv, r);
-          lk.lock();
-          sv->used_bytes -= r->bytes;
-          if (r->state != EQF_REQ_FAILED) r->state = EQF_REQ_CANCELLED;
-          it = sv->active.erase(it);
-          if (r->forget) sv->reqs.erase(r->ticket);
-        } else {
-          ++it;
-        }
-      }
-      act = sv->active;
+    bool progress = false;
+    for (Lane& L : sv->lanes) {
+      if (!L.busy) continue;
+      const hipError_t q = hipEventQuery(L.ev);
+      if (q == hipErrorNotReady) continue;
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (q != hipSuccess) sv->thread_error = hipGetErrorString(q);
+      lane_tick_done(sv, L, sv->now_ms());
+      progress = true;
     }
-    // Launch one frame of every resident scene: the wavefront-mode scenes in
-    // one fused launch (largest first), the red-black ones by their launch
-    // groups (one per pair of iteration counts, one graph replay each).
-    bool launched = false, launched_rb = false;
-    double t_launch = sv->now_ms();
-    std::vector<Req*> wf, rbs;
-    for (Req* r : act) (r->rb ? rbs : wf).push_back(r);
-    if (!wf.empty()) {
-      std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
-      int maxns = 1;
-      for (Req* r : wf) maxns = std::max(maxns, r->scene->ns);
-      std::vector<uint64_t> uids(wf.size());
-      for (size_t i = 0; i < wf.size(); ++i) uids[i] = wf[i]->scene->uid;
-      hipError_t e = hipSuccess;
-      if (uids != last_uids) {
-        descs.resize(wf.size());
-        for (size_t i = 0; i < wf.size(); ++i) {
-          descs[i] = wf[i]->scene->desc;
-          descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
-        }
-        last_uids.clear();
-        if ((int)wf.size() > sv->desc_cap) {
-          const int cap = std::max<int>((int)wf.size(), 2 * sv->desc_cap);
-          (void)hipFree(sv->d_descs);
-          sv->d_descs = nullptr;
-          sv->desc_cap = 0;
-          e = hipMalloc(&sv->d_descs, sizeof(SceneDesc) * cap);
-          if (e == hipSuccess) sv->desc_cap = cap;
-          else sv->d_descs = nullptr;
-        }
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(sv->d_descs, descs.data(), sizeof(SceneDesc) * wf.size(), hipMemcpyHostToDevice,
-                             sv->stream);
-        if (e == hipSuccess) last_uids = uids;
-      }
-      int lds = 0, waves = 1;
-      for (Req* r : wf) {
-        lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
-        waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
-      }
-      if (e == hipSuccess) e = eqf_internal_launch(sv->d_descs, (int)wf.size(), waves, lds, 1, sv->stream);
-      if (e != hipSuccess) {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        sv->thread_error = hipGetErrorString(e);
-      } else {
-        launched = true;
-      }
-    }
-    {
-      std::map<std::pair<int, int>, std::vector<Req*>> by_iters;
-      for (Req* r : rbs) by_iters[{r->scene->desc.diff_iters, r->scene->desc.press_iters}].push_back(r);
-      for (auto& [key, reqs] : by_iters) {
-        std::stable_sort(reqs.begin(), reqs.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
-        std::vector<eqf_scene*> members;
-        for (Req* r : reqs) members.push_back(r->scene);
-        std::string err;
-        void*& g = sv->rb_groups[key];
-        if (!g) {
-          g = rb_batch_create(members, sv->rstream[sv->rb_next], err);
-          sv->rb_next = (sv->rb_next + 1) % eqf_server::kRbStreams;
-        }
-        int rc = g ? EQF_OK : EQF_EINVAL;
-        if (g) {
-          rb_batch_set_scenes(g, members);
-          rc = rb_batch_step(g, 1, err);
-        }
-        if (rc != EQF_OK) {
-          std::lock_guard<std::mutex> lk(sv->mu);
-          sv->thread_error = err;
-          for (Req* r : reqs) {
-            r->state = EQF_REQ_FAILED;
-            r->cancel = true;  // dropped at the next tick
-          }
-        } else {
-          launched_rb = true;
-        }
-      }
-    }
-    auto tick_running = [&] {
-      if (launched && hipStreamQuery(sv->stream) == hipErrorNotReady) return true;
-      for (hipStream_t q : sv->rstream)
-        if (launched_rb && hipStreamQuery(q) == hipErrorNotReady) return true;
-      return false;
-    };
-    // Admit queued requests while the tick runs: at least one per tick, then
-    // more only while the launch is still in flight (with nothing resident,
-    // for up to 50 ms), so admission never delays the resident scenes' ticks.
-    std::vector<Req*> admitted;
-    const double t_adm0 = sv->now_ms();
-    for (;;) {
-      if (!admitted.empty()) {
-        if ((launched || launched_rb) && !tick_running()) break;
-        if (!launched && !launched_rb && sv->now_ms() - t_adm0 > 50.0) break;
-      }
-      Req* r = nullptr;
+    for (int li = 0; li < NLANES; ++li) {
+      Lane& L = sv->lanes[li];
+      if (L.busy) continue;
       {
         std::lock_guard<std::mutex> lk(sv->mu);
-        if (sv->queue.empty()) break;
-        Req* h = sv->queue.front();
-        if (h->cancel) {
-          sv->queue.pop_front();
-          h->state = EQF_REQ_CANCELLED;
-          if (h->forget) sv->reqs.erase(h->ticket);
-          continue;
+        bool moved = false;
+        for (auto it = L.members.begin(); it != L.members.end();) {
+          Req* r = *it;
+          if (r->cancel || r->frames_done >= r->nframes) {
+            sv->retiring.push_back(r);
+            it = L.members.erase(it);
+            moved = true;
+          } else {
+            ++it;
+          }
         }
-        const size_t resident = sv->active.size() + admitted.size();
-        if (sv->cfg.max_active > 0 && resident >= (size_t)sv->cfg.max_active) break;
-        if (sv->used_bytes + h->bytes > sv->cfg.hbm_budget_bytes) break;
-        sv->queue.pop_front();
-        sv->used_bytes += h->bytes;
-        sv->peak_bytes = std::max(sv->peak_bytes, sv->used_bytes);
-        r = h;
+        if (moved) sv->cv_work.notify_all();
+        L.members.insert(L.members.end(), sv->incoming[li].begin(), sv->incoming[li].end());
+        sv->incoming[li].clear();
       }
-      std::string err;
-      if (admit(sv, r, err) != EQF_OK) {
+      if (L.members.empty()) continue;
+      const std::string err = lane_launch(sv, li);
+      if (!err.empty()) {
         std::lock_guard<std::mutex> lk(sv->mu);
-        sv->used_bytes -= r->bytes;
-        r->state = EQF_REQ_FAILED;
         sv->thread_error = err;
-        continue;
-      }
-      r->t_admit = sv->now_ms();
-      admitted.push_back(r);
-    }
-    double tick_ms = 0;
-    if (launched) (void)hipStreamSynchronize(sv->stream);
-    if (launched_rb)
-      for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
-    if (launched || launched_rb) tick_ms = sv->now_ms() - t_launch;
-    // Retire finished scenes, then add the admitted ones.
-    std::vector<Req*> done;
-    {
-      std::lock_guard<std::mutex> lk(sv->mu);
-      if (launched || launched_rb) {
-        std::vector<Req*> ran = launched ? wf : std::vector<Req*>();
-        for (Req* r : rbs)
-          if (!r->cancel) ran.push_back(r);
-        sv->ticks++;
-        sv->scene_frames += (int64_t)ran.size();
-        sv->busy_ms += tick_ms;
-        sv->tick_lat.emplace_back(tick_ms, (int)ran.size());
-        for (Req* r : ran) {
-          r->frames_done++;
-          sv->cell_updates += (double)r->cells;
+        for (Req* r : L.members) {
+          r->state = EQF_REQ_FAILED;
+          r->cancel = true;  // retired at the next pass
         }
       }
-      for (auto it = sv->active.begin(); it != sv->active.end();) {
-        if ((*it)->frames_done >= (*it)->nframes) {
-          done.push_back(*it);
-          it = sv->active.erase(it);
-        } else {
-          ++it;
-        }
-      }
+      progress = true;
     }
-    for (Req* r : done) finish(sv, r);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (sv->stop) break;
+    if (!progress) {
+      bool any = false;
+      for (const Lane& L : sv->lanes) any = any || L.busy || !L.members.empty();
+      for (const auto& v : sv->incoming) any = any || !v.empty();
+      if (any)  // poll the running ticks
+        sv->cv_tick.wait_for(lk, std::chrono::microseconds(100));
+      else
+        sv->cv_tick.wait(lk, [sv] {
+          if (sv->stop) return true;
+          for (const auto& v : sv->incoming)
+            if (!v.empty()) return true;
+          return false;
+        });
+    }
+  }
+}
+
+// Admission thread: reads back and frees retired scenes, then admits queued
+// requests one at a time while their memory fits the budget.
+static void work_loop(eqf_server* sv) {
+  (void)hipSetDevice(sv->ctx->device);
+  for (;;) {
+    std::vector<Req*> ret;
     {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      auto ready = [sv] {
+        if (sv->stop || !sv->retiring.empty()) return true;
+        if (sv->queue.empty()) return false;
+        const Req* h = sv->queue.front();
+        if (h->cancel) return true;
+        if (sv->cfg.max_active > 0 && sv->active.size() >= (size_t)sv->cfg.max_active) return false;
+        return sv->used_bytes + h->bytes <= sv->cfg.hbm_budget_bytes;
+      };
+      if (sv->queue.empty() && sv->active.empty()) sv->cv_idle.notify_all();
+      sv->cv_work.wait(lk, ready);
+      if (sv->stop) break;
+      ret.swap(sv->retiring);
+    }
+    for (Req* r : ret) finish(sv, r);
+    if (!ret.empty()) {
       std::lock_guard<std::mutex> lk(sv->mu);
-      for (Req* r : done) {
+      for (Req* r : ret) {
+        // leaves `active` only with its final state, so eqf_server_wait_idle
+        // never returns while a request still reads as resident
+        sv->active.erase(std::find(sv->active.begin(), sv->active.end(), r));
         sv->used_bytes -= r->bytes;
         if (r->state != EQF_REQ_FAILED) r->state = r->cancel ? EQF_REQ_CANCELLED : EQF_REQ_DONE;
         r->t_done = sv->now_ms();
         if (r->forget) sv->reqs.erase(r->ticket);
       }
-      for (Req* r : admitted) {
-        // Staging buffers are only needed again for the final read-back; free
-        // them now that no tick is running (hipFree waits for the device).
-        eqf_dev_trim(r->scene);
-        if (r->nframes <= 0) {  // nothing to run
-          finish(sv, r);
-          sv->used_bytes -= r->bytes;
-          r->state = r->cancel ? EQF_REQ_CANCELLED : EQF_REQ_DONE;
-          r->t_done = sv->now_ms();
-          if (r->forget) sv->reqs.erase(r->ticket);
-          continue;
-        }
-        r->state = EQF_REQ_ACTIVE;
-        sv->active.push_back(r);
-        sv->queue_lat.push_back(r->t_admit - r->t_submit);
-      }
-      sv->max_resident = std::max<int64_t>(sv->max_resident, (int64_t)sv->active.size());
-      if (sv->queue.empty() && sv->active.empty()) sv->cv_idle.notify_all();
     }
+    admit_one(sv);
   }
-  // shutdown: free everything still resident
-  std::lock_guard<std::mutex> lk(sv->mu);
-  for (Req* r : sv->active) {
-    eqf_scene_destroy(r->scene);
-    r->scene = nullptr;
-    r->state = EQF_REQ_CANCELLED;
-  }
-  sv->active.clear();
-  sv->cv_idle.notify_all();
 }
 
 extern "C" int eqf_server_create(eqf_ctx* ctx, const eqf_server_config* cfg, eqf_server** out) {
@@ -394,17 +443,35 @@ extern "C" int eqf_server_create(eqf_ctx* ctx, const eqf_server_config* cfg, eqf
   sv->ctx = ctx;
   sv->cfg = *cfg;
   sv->t0 = Clock::now();
-  bool ok = hipStreamCreate(&sv->stream) == hipSuccess && hipStreamCreate(&sv->astream) == hipSuccess;
-  for (hipStream_t& q : sv->rstream) ok = ok && hipStreamCreate(&q) == hipSuccess;
+  bool ok = hipStreamCreate(&sv->astream) == hipSuccess;
+  if (ok) {
+    hipMemPoolProps props;
+    std::memset(&props, 0, sizeof(props));
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = ctx->device;
+    ok = hipMemPoolCreate(&sv->pool, &props) == hipSuccess;
+    // keep freed memory in the pool for the next admissions (the budget bounds it)
+    uint64_t keep = UINT64_MAX;
+    ok = ok && hipMemPoolSetAttribute(sv->pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+  }
+  for (Lane& L : sv->lanes) {
+    ok = ok && hipStreamCreate(&L.st) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) == hipSuccess;
+  }
   if (!ok) {
-    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    if (sv->pool) (void)hipMemPoolDestroy(sv->pool);
     if (sv->astream) (void)hipStreamDestroy(sv->astream);
-    for (hipStream_t q : sv->rstream)
-      if (q) (void)hipStreamDestroy(q);
+    for (Lane& L : sv->lanes) {
+      if (L.st) (void)hipStreamDestroy(L.st);
+      if (L.ev) (void)hipEventDestroy(L.ev);
+    }
     delete sv;
     return eqf_internal_error(EQF_EDEVICE, "hipStreamCreate failed");
   }
-  sv->th = std::thread(server_loop, sv);
+  sv->th_tick = std::thread(tick_loop, sv);
+  sv->th_work = std::thread(work_loop, sv);
   *out = sv;
   return EQF_OK;
 }
@@ -446,7 +513,7 @@ extern "C" int eqf_server_submit(eqf_server* sv, const eqf_request* req, uint64_
   *ticket = r->ticket;
   sv->queue.push_back(r.get());
   sv->reqs.emplace(r->ticket, std::move(r));
-  sv->cv.notify_all();
+  sv->cv_work.notify_all();
   return EQF_OK;
 }
 
@@ -485,9 +552,10 @@ extern "C" int eqf_server_release(eqf_server* sv, uint64_t ticket) {
   if (it == sv->reqs.end()) return eqf_internal_error(EQF_ENOTFOUND, "unknown ticket");
   Req* r = it->second.get();
   if (r->state == EQF_REQ_QUEUED || r->state == EQF_REQ_ACTIVE) {
-    r->cancel = true;  // the server thread retires it and forgets the ticket
+    r->cancel = true;  // the server threads retire it and forget the ticket
     r->forget = true;
-    sv->cv.notify_all();
+    sv->cv_work.notify_all();
+    sv->cv_tick.notify_all();
     return EQF_OK;
   }
   sv->reqs.erase(it);
@@ -521,22 +589,43 @@ static double pct_weighted(std::vector<std::pair<double, int>> v, double q) {
   return v.back().first;
 }
 
-extern "C" int eqf_server_stats_get(eqf_server* sv, eqf_server_stats* st) {
-  if (!sv || !st) return eqf_internal_error(EQF_EINVAL, "NULL argument");
-  std::lock_guard<std::mutex> lk(sv->mu);
+// Statistics over the scenes of sizes [lo, hi] (every scene: 0, UINT32_MAX).
+static void stats_for(eqf_server* sv, uint32_t lo, uint32_t hi, eqf_server_stats* st) {
   std::memset(st, 0, sizeof(*st));
+  std::vector<std::pair<double, int>> f, q;
+  double cells = 0;
+  for (auto& x : sv->frame_lat)
+    if (x.first >= lo && x.first <= hi) {
+      f.emplace_back(x.second, 1);
+      cells += (double)(x.first - 2) * (double)(x.first - 2);
+    }
+  for (auto& x : sv->queue_lat)
+    if (x.first >= lo && x.first <= hi) q.emplace_back(x.second, 1);
   st->ticks = sv->ticks;
-  st->scene_frames = sv->scene_frames;
-  st->cell_updates = sv->cell_updates;
-  st->busy_ms = sv->busy_ms;
-  st->p50_frame_ms = pct_weighted(sv->tick_lat, 0.50);
-  st->p99_frame_ms = pct_weighted(sv->tick_lat, 0.99);
-  std::vector<std::pair<double, int>> q;
-  for (double x : sv->queue_lat) q.emplace_back(x, 1);
+  st->scene_frames = (int64_t)f.size();
+  st->cell_updates = cells;
+  // wall time with at least one lane running, the current stretch included
+  st->busy_ms = sv->busy_ms + (sv->busy_lanes > 0 ? sv->now_ms() - sv->busy_since : 0.0);
+  st->p50_frame_ms = pct_weighted(f, 0.50);
+  st->p99_frame_ms = pct_weighted(f, 0.99);
   st->p50_queue_ms = pct_weighted(q, 0.50);
   st->p99_queue_ms = pct_weighted(q, 0.99);
   st->max_resident = sv->max_resident;
   st->peak_bytes = sv->peak_bytes;
+}
+
+extern "C" int eqf_server_stats_get(eqf_server* sv, eqf_server_stats* st) {
+  if (!sv || !st) return eqf_internal_error(EQF_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  stats_for(sv, 0, UINT32_MAX, st);
+  return EQF_OK;
+}
+
+extern "C" int eqf_server_stats_sizes(eqf_server* sv, uint32_t min_size, uint32_t max_size, eqf_server_stats* st) {
+  if (!sv || !st) return eqf_internal_error(EQF_EINVAL, "NULL argument");
+  if (min_size > max_size) return eqf_internal_error(EQF_EINVAL, "min_size > max_size");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  stats_for(sv, min_size, max_size, st);
   return EQF_OK;
 }
 
@@ -545,19 +634,36 @@ extern "C" int eqf_server_destroy(eqf_server* sv) {
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     sv->stop = true;
-    sv->cv.notify_all();
+    sv->cv_tick.notify_all();
+    sv->cv_work.notify_all();
   }
-  if (sv->th.joinable()) sv->th.join();
-  (void)hipStreamSynchronize(sv->stream);
-  for (auto& kv : sv->rb_groups) {
-    for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
-    rb_batch_destroy(kv.second);
+  if (sv->th_tick.joinable()) sv->th_tick.join();
+  if (sv->th_work.joinable()) sv->th_work.join();
+  // wait for the ticks in flight, then free every scene still admitted
+  for (Lane& L : sv->lanes)
+    if (L.busy) (void)hipEventSynchronize(L.ev);
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (Req* r : sv->active) {
+      eqf_scene_destroy(r->scene);
+      r->scene = nullptr;
+      r->state = EQF_REQ_CANCELLED;
+    }
+    sv->active.clear();
+    sv->retiring.clear();
+    for (auto& v : sv->incoming) v.clear();
+    sv->cv_idle.notify_all();
   }
-  for (hipStream_t q : sv->rstream) (void)hipStreamSynchronize(q);
-  (void)hipFree(sv->d_descs);
-  (void)hipStreamDestroy(sv->stream);
+  for (Lane& L : sv->lanes) (void)hipStreamSynchronize(L.st);
+  for (auto& kv : sv->rb_groups) rb_batch_destroy(kv.second);
+  for (Lane& L : sv->lanes) {
+    (void)hipFree(L.d_descs);
+    (void)hipStreamDestroy(L.st);
+    (void)hipEventDestroy(L.ev);
+  }
+  (void)hipStreamSynchronize(sv->astream);
   (void)hipStreamDestroy(sv->astream);
-  for (hipStream_t q : sv->rstream) (void)hipStreamDestroy(q);
+  (void)hipMemPoolDestroy(sv->pool);
   delete sv;
   return EQF_OK;
 }

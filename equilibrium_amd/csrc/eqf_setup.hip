@@ -69,18 +69,31 @@ __global__ void k_to_rm(Geo g, const float* __restrict__ src, float* __restrict_
 }
 
 // rowf[j] / colf[i]: row / column holds a NoWall cell (Passive copy rule of
-// fluid.rs:179-187).
-__global__ void k_rowcol(Geo g, const uint8_t* __restrict__ m, uint8_t* __restrict__ rowf,
-                         uint8_t* __restrict__ colf) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.S) return;
-  uint8_t r = 0, c = 0;
-  for (int t = 0; t < g.S; ++t) {
-    r |= m[(int64_t)k * g.S + t] != EQF_DEFAULT_WALL;
-    c |= m[(int64_t)t * g.S + k] != EQF_DEFAULT_WALL;
+// fluid.rs:179-187).  A 2D grid of 256-column x 64-row tiles: each thread ORs
+// its column over the tile's rows (coalesced row reads), each wave ORs every
+// row over its 64 columns with a wave vote, and tiles combine through atomic
+// ORs into zeroed 32-bit flags; k_rowcol_bytes turns them into the byte arrays.
+__global__ void __launch_bounds__(256) k_rowcol_or(Geo g, const uint8_t* __restrict__ m, uint32_t* __restrict__ rowf32,
+                                                   uint32_t* __restrict__ colf32) {
+  const int S = g.S;
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y0 = blockIdx.y * 64;
+  const int y1 = min(y0 + 64, S);
+  uint32_t c = 0;
+  for (int y = y0; y < y1; ++y) {  // uniform bounds: every lane takes part in the vote
+    const bool f = x < S && m[(int64_t)y * S + x] != EQF_DEFAULT_WALL;
+    c |= f ? 1u : 0u;
+    if (__any(f) && (threadIdx.x & 63) == 0) atomicOr(&rowf32[y], 1u);
   }
-  rowf[k] = r;
-  colf[k] = c;
+  if (c) atomicOr(&colf32[x], 1u);
+}
+
+__global__ void k_rowcol_bytes(int S, const uint32_t* __restrict__ rowf32, const uint32_t* __restrict__ colf32,
+                               uint8_t* __restrict__ rowf, uint8_t* __restrict__ colf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= S) return;
+  rowf[k] = rowf32[k] != 0;
+  colf[k] = colf32[k] != 0;
 }
 
 // Per-cell boundary codes (eqf_layout.h CodeBits) and the set_boundaries
@@ -178,7 +191,7 @@ int eqf_dev_init_fields(eqf_scene* s, int times) {
 
 static int ensure_stage(eqf_scene* s) {
   if (s->d_stage) return EQF_OK;
-  hipError_t e = hipMalloc(&s->d_stage, sizeof(float) * (size_t)s->S * s->S + 256);
+  hipError_t e = eqf_scene_malloc(s, (void**)&s->d_stage, sizeof(float) * (size_t)s->S * s->S + 256);
   return e == hipSuccess ? EQF_OK
                          : eqf_internal_error(EQF_ENOMEM, std::string("staging buffer: ") + hipGetErrorString(e));
 }
@@ -231,25 +244,47 @@ int eqf_dev_download(eqf_scene* s, const float* src, float* host) {
 }
 
 void eqf_dev_trim(eqf_scene* s) {
-  (void)hipFree(s->d_stage);
+  eqf_scene_mfree(s, s->d_stage);
   s->d_stage = nullptr;
-  (void)hipFree(s->d_mask);
+  eqf_scene_mfree(s, s->d_mask);
   s->d_mask = nullptr;
+}
+
+// Layout of the scene's d_mask buffer: the mask (S*S bytes), colf (S bytes),
+// then the 32-bit row/column flags (2*S) and the entry counters of the two
+// k_codes passes (3 + 3), zeroed by one memset.
+struct MaskBuf {
+  size_t colf, flags, cnt, bytes;
+};
+static MaskBuf mask_buf(int S) {
+  const size_t n = (size_t)S * S;
+  auto up = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  MaskBuf b;
+  b.colf = up(n);
+  b.flags = up(b.colf + (size_t)S);
+  b.cnt = b.flags + 8 * (size_t)S;
+  b.bytes = b.cnt + 6 * sizeof(int);
+  return b;
 }
 
 int eqf_dev_rebuild_bnd(eqf_scene* s) {
   const int S = s->S;
   const int64_t n = (int64_t)S * S;
+  const MaskBuf mb = mask_buf(S);
   hipError_t e = hipSuccess;
-  if (!s->d_mask) e = hipMalloc(&s->d_mask, (size_t)n + 2 * (size_t)S + 64);
+  if (!s->d_mask) e = eqf_scene_malloc(s, (void**)&s->d_mask, mb.bytes);
   if (e != hipSuccess) return eqf_internal_error(EQF_ENOMEM, std::string("mask buffer: ") + hipGetErrorString(e));
-  uint8_t* d_colf = s->d_mask + n;
-  int* d_cnt = (int*)(s->d_mask + ((n + 2 * S + 15) & ~(int64_t)15));
+  uint8_t* d_colf = s->d_mask + mb.colf;
+  uint32_t* d_flags = (uint32_t*)(s->d_mask + mb.flags);
+  int* d_cnt = (int*)(s->d_mask + mb.cnt);
   const Geo g = geo_of(s);
   e = hipMemcpyAsync(s->d_mask, s->mask.data(), (size_t)n, hipMemcpyHostToDevice, s->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, 3 * sizeof(int), s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_flags, 0, mb.bytes - mb.flags, s->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_rowcol, dim3(blocks_for(S)), dim3(256), 0, s->stream, g, s->d_mask, s->d_rowf, d_colf);
+    hipLaunchKernelGGL(k_rowcol_or, dim3((unsigned)((S + 255) / 256), (unsigned)((S + 63) / 64)), dim3(256), 0,
+                       s->stream, g, s->d_mask, d_flags, d_flags + S);
+    hipLaunchKernelGGL(k_rowcol_bytes, dim3(blocks_for(S)), dim3(256), 0, s->stream, S, d_flags, d_flags + S,
+                       s->d_rowf, d_colf);
     hipLaunchKernelGGL(k_codes, dim3(blocks_for(n)), dim3(256), 0, s->stream, g, s->d_mask, s->d_rowf, d_colf,
                        s->d_code, d_cnt, (int2*)nullptr, (int2*)nullptr, (int2*)nullptr, 0);
     e = hipGetLastError();
@@ -260,20 +295,19 @@ int eqf_dev_rebuild_bnd(eqf_scene* s) {
   if (e != hipSuccess) return eqf_internal_error(EQF_EDEVICE, std::string("boundary codes: ") + hipGetErrorString(e));
   const size_t total = std::max<size_t>(1, (size_t)cnt[0] + cnt[1] + cnt[2]);
   if (total > s->bnd_cap) {
-    (void)hipFree(s->d_bnd);
+    eqf_scene_mfree(s, s->d_bnd);
     s->d_bnd = nullptr;
-    e = hipMalloc(&s->d_bnd, sizeof(int2) * total);
+    e = eqf_scene_malloc(s, (void**)&s->d_bnd, sizeof(int2) * total);
     if (e != hipSuccess) return eqf_internal_error(EQF_ENOMEM, std::string("boundary lists: ") + hipGetErrorString(e));
     s->bnd_cap = total;
   }
-  e = hipMemsetAsync(d_cnt, 0, 3 * sizeof(int), s->stream);
-  if (e == hipSuccess) {
+  {
     const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)cyc_code_bytes(s->P, s->ns) : (size_t)n;
     e = hipMemsetAsync(s->d_code, 0, ncode, s->stream);
   }
-  if (e == hipSuccess) {
+  if (e == hipSuccess) {  // the fill pass counts into the second, still zero, counters
     hipLaunchKernelGGL(k_codes, dim3(blocks_for(n)), dim3(256), 0, s->stream, g, s->d_mask, s->d_rowf, d_colf,
-                       s->d_code, d_cnt, s->d_bnd, s->d_bnd + cnt[0], s->d_bnd + cnt[0] + cnt[1], 1);
+                       s->d_code, d_cnt + 3, s->d_bnd, s->d_bnd + cnt[0], s->d_bnd + cnt[0] + cnt[1], 1);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

@@ -241,6 +241,10 @@ int eqf_server_release(eqf_server *sv, uint64_t ticket);
 /* Waits until no request is queued or resident (timeout_ms < 0: forever). */
 int eqf_server_wait_idle(eqf_server *sv, double timeout_ms);
 int eqf_server_stats_get(eqf_server *sv, eqf_server_stats *st);
+/* The same statistics restricted to requests of size min_size..max_size
+ * (frame and queue latency percentiles, scene_frames, cell_updates); ticks,
+ * busy_ms, max_resident and peak_bytes stay server-wide. */
+int eqf_server_stats_sizes(eqf_server *sv, uint32_t min_size, uint32_t max_size, eqf_server_stats *st);
 int eqf_server_destroy(eqf_server *sv);
 
 /* ---- the renderer's side: RGBA image and per-frame snapshots ---------- */
