@@ -46,13 +46,43 @@ def b_alg(iters):
     return 60 * iters + 108
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch measured by the committed rocprofv3 PMC runs."""
+def profiled_traffic(kernel, shape=None):
+    """HBM bytes per launch measured by the committed rocprofv3 PMC runs, for
+    the launch shape ("max_waves:order") the run used; None when no profile
+    was taken at that shape.  Returns (bytes, profile entry)."""
     try:
         with open(os.path.join(REPO, "profiles", "traffic.json")) as f:
-            return json.load(f)[kernel]["traffic_bytes_per_launch"]
+            ent = json.load(f)[kernel]
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
+    if "by_shape" in ent:
+        ent = ent["by_shape"].get(shape)
+    elif ent.get("shape") != shape:
+        ent = None
+    if ent is None:
+        return None, None
+    return ent.get("traffic_bytes_per_launch"), ent
+
+
+def bench_bandwidth(gib=8, blocks_per_cu=(4, 8), reps=5):
+    """Practical HBM ceiling: hand-written streaming read / write / copy
+    kernels (libeqf_bwprobe.so) on a working set of `gib` GiB, far above the
+    256 MB MALL; the best of the launch shapes tried."""
+    import ctypes
+    path = os.path.join(REPO, "equilibrium_amd", "libeqf_bwprobe.so")
+    L = ctypes.CDLL(path)
+    L.eqf_bwprobe_run.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_double)]
+    best = {"read": 0.0, "write": 0.0, "copy": 0.0}
+    for bpc in blocks_per_cu:
+        o = (ctypes.c_double * 3)()
+        rc = L.eqf_bwprobe_run(0, gib << 30, bpc, reps, o)
+        if rc != 0:
+            raise RuntimeError(f"bandwidth probe failed: HIP error {rc}")
+        for k, v in zip(("read", "write", "copy"), o):
+            best[k] = max(best[k], v)
+    return {"unit": "GB/s", "working_set_gib": gib, **best,
+            "kernels": "float4 non-temporal streams, 8 accesses in flight per thread, grid-stride"}
 
 
 def log(*a):
@@ -102,9 +132,36 @@ def bench_batch(scs, steps, warmup, tuning=None):
     wall = time.perf_counter() - t0
     ev_ms = b.last_ms()
     b.close()
+    # one scene per size, checked bit for bit against the oracle in the CPU leg
+    picks = {}
+    for i, sc in enumerate(scs):
+        picks.setdefault(sc["size"], i)
+    CHECKS["batch"] = ([(scs[i], {name: hs[i].download(fid) for fid, name in enumerate(
+        ("density", "scratch", "vx", "vy", "vx0", "vy0"))}) for i in sorted(picks.values())], warmup + steps)
     for h in hs:
         h.close()
     return wall, ev_ms, {"max_waves": w, "order": "largest first" if o == 0 else "given", "trial_ms": timings}
+
+
+def check_batch_exact(samples, frames, threads):
+    """The timed batch's scenes (one per size) against the oracle after all
+    the frames the batch ran: every field, bit for bit."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from helpers import oracle_state
+
+    def one(item):
+        sc, got = item
+        st = oracle_state(sc)
+        st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], frames,
+               src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+        bad = {k: int(np.sum(got[k].view(np.uint32) != st.f[k].view(np.uint32))) for k in got}
+        return {"N": sc["size"] - 2, "cells_differing": sum(bad.values())}
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(min(threads, len(samples))) as ex:
+        res = list(ex.map(one, samples))
+    return {"frames": frames, "scenes": res, "bit_exact": all(r["cells_differing"] == 0 for r in res),
+            "oracle_seconds": time.perf_counter() - t0}
 
 
 def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
@@ -147,48 +204,69 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
            "sweeps_per_launch": tsw, "roofline_alg_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
                HBM_PEAK_GBS * 1e9)}
     if check_frames > 0:
-        # GPU fields after the first frames of one scene per size, checked
-        # against the f64 oracle in the CPU leg
+        # GPU fields after each of the first frames of one scene per size,
+        # checked against the f64 oracle in the CPU leg
         got = {}
         for N in sorted({sc["size"] - 2 for sc in scs}):
             sc = [x for x in scs if x["size"] - 2 == N][0]
             g = mk(sc)
-            g.step(check_frames)
-            g.sync()
-            got[N] = (sc, {name: g.download(fid) for name, fid in
-                           (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY))})
+            frames = []
+            for _ in range(check_frames):
+                g.step(1)
+                g.sync()
+                frames.append({name: g.download(fid) for name, fid in
+                               (("density", _abi.DENSITY), ("vx", _abi.VX), ("vy", _abi.VY))})
+            got[N] = (sc, frames)
             g.close()
         CHECKS["rbmix"] = (res, check_frames, got)
     return res
 
 
-def check_batch_redblack(res, check_frames, got):
-    """vs_fp64 of the red-black batch leg: the GPU fields after the first
-    frames against the f64 red-black oracle, beside the reference's own f32
-    step (lexicographic order) against its f64 version on the same scene."""
+def _rel(a, b):
+    a = a.astype(np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def check_batch_redblack(res, check_frames, got, threads=16):
+    """vs_fp64 of the red-black batch leg: the GPU fields after each of the
+    first frames against the f64 red-black oracle, beside the reference's own
+    f32 step (lexicographic order) against its f64 version on the same scene
+    and frames.  Norm-wise relative error per field and frame."""
     import oracle
 
-    def oracle_run(sc, dtype, order):
-        st = oracle.State.new(sc["size"]).astype(dtype)
-        for r in sc["rects"]:
-            st.fill_rect(*r)
-        st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], check_frames, order=order,
-               src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
-        return st
+    def states(sc):
+        out = []
+        for dtype, order in ((np.float64, oracle.RB), (np.float32, oracle.LEX), (np.float64, oracle.LEX)):
+            st = oracle.State.new(sc["size"]).astype(dtype)
+            for r in sc["rects"]:
+                st.fill_rect(*r)
+            out.append((st, order))
+        return out
 
-    def rel(a, b):
-        a = a.astype(np.float64)
-        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    def one(N):
+        sc, frames = got[N]
+        sts = states(sc)
+        ours, ref = [], []
+        for fields in frames:
+            for st, order in sts:
+                st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], 1, order=order,
+                       src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+            rb64, lex32, lex64 = (st for st, _ in sts)
+            ours.append(max(_rel(fields[k], rb64.f[k]) for k in fields))
+            ref.append(max(_rel(lex32.f[k], lex64.f[k]) for k in fields))
+        return N, ours, ref
 
-    errs, ref_errs = {}, {}
-    for N, (sc, fields) in got.items():
-        st = oracle_run(sc, np.float64, oracle.RB)
-        lex32, lex64 = oracle_run(sc, np.float32, oracle.LEX), oracle_run(sc, np.float64, oracle.LEX)
-        errs[f"N={N}"] = {name: rel(fields[name], st.f[name]) for name in fields}
-        ref_errs[f"N={N}"] = {name: rel(lex32.f[name], lex64.f[name]) for name in fields}
-    res["vs_fp64"] = {"frames": check_frames, "rel_norm": errs,
-                      "worst_rel_norm": max(max(v.values()) for v in errs.values()),
-                      "reference_f32_vs_fp64_rel_norm": ref_errs}
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(min(threads, len(got))) as ex:
+        rows = list(ex.map(one, sorted(got)))
+    res["vs_fp64"] = {
+        "frames": check_frames,
+        "metric": "max over density, vx, vy of ||gpu - f64|| / ||f64|| (norm-wise), per frame",
+        "per_frame": {f"N={N}": [float(f"{x:.3g}") for x in ours] for N, ours, _ in rows},
+        "reference_f32_vs_fp64_per_frame": {f"N={N}": [float(f"{x:.3g}") for x in ref] for N, _, ref in rows},
+        "worst_rel_norm": max(max(o) for _, o, _ in rows),
+        "worst_reference_f32_rel_norm": max(max(r) for _, _, r in rows),
+        "oracle_seconds": time.perf_counter() - t0}
 
 
 def bench_single_dropin(scs, frames):
@@ -353,9 +431,10 @@ def check_redblack_4096(res, sc, frames, p_gpu, div_gpu):
         for name, got in fields.items():
             got = got.astype(np.float64)
             want = st.f[name]
-            e[name] = {"max_abs": float(np.max(np.abs(got - want))),
-                       "rel_norm": float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))}
+            e[name] = {"max_abs": float(f"{np.max(np.abs(got - want)):.3g}"),
+                       "rel_norm": float(f"{np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300):.3g}")}
         errs.append(e)
+        frames[f] = None  # free the frame's fields
         log(f"rb N={N} frame {f}: {e}")
     mx, rr, rd = oracle.poisson_residual(p_gpu.astype(np.float64), div_gpu.astype(np.float64), N + 2)
     mx64, rr64, rd64 = oracle.poisson_residual(st.f["vx0"], st.f["vy0"], N + 2)
@@ -461,6 +540,13 @@ def bench_serving(rates, seconds, budget_gb=200.0, n_max=2048, buckets=16):
         t_end = time.perf_counter() - t0
         time.sleep(0.5)
         st = srv.stats()
+        classes = {}
+        for name, lo, hi in (("N<=64", 0, 66), ("N<=128", 67, 130), ("N<=256", 131, 258), ("N<=512", 259, 514),
+                             ("N<=1024", 515, 1026), ("N>1024 (red-black)", 1027, 1 << 31)):
+            c = srv.stats(lo, hi)
+            classes[name] = {"p50_frame_ms": round(c["p50_frame_ms"], 3), "p99_frame_ms": round(c["p99_frame_ms"], 3),
+                             "p50_queue_ms": round(c["p50_queue_ms"], 3), "p99_queue_ms": round(c["p99_queue_ms"], 3),
+                             "scene_frames": c["scene_frames"]}
         for t in tickets:
             srv.release(t)
         srv.wait_idle(120000)
@@ -471,7 +557,8 @@ def bench_serving(rates, seconds, budget_gb=200.0, n_max=2048, buckets=16):
              "p50_queue_ms": st["p50_queue_ms"], "p99_queue_ms": st["p99_queue_ms"],
              "scene_frames": st["scene_frames"], "ticks": st["ticks"],
              "max_resident": st["max_resident"], "peak_gb": st["peak_bytes"] / 2 ** 30,
-             "red_black_share": sum(1 for r in reqs if r[1] > WAVEFRONT_MAX_N) / max(1, len(reqs))}
+             "red_black_share": sum(1 for r in reqs if r[1] > WAVEFRONT_MAX_N) / max(1, len(reqs)),
+             "by_size": classes}
         log(f"serve rate {rate}/s: {r}")
         srv.close()
         out.append(r)
@@ -485,13 +572,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--no-extras", action="store_true", help="headline batch only (for profiling)")
-    ap.add_argument("--legs", type=str, default="batch,rbmix,single,n1024,rb4096,snap,serve,cpu",
+    ap.add_argument("--legs", type=str, default="bw,batch,rbmix,single,n1024,rb4096,snap,serve,cpu",
                     help="comma list of legs to run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--rb-frames", type=int, default=10)
-    ap.add_argument("--rb-check-frames", type=int, default=2)
+    ap.add_argument("--rb-check-frames", type=int, default=20)
     ap.add_argument("--batch-tuning", type=str, default="",
                     help="max_waves:order for the batch instead of autotuning (e.g. 16:0)")
     ap.add_argument("--rb-tiling", type=str, default="",
@@ -523,6 +610,11 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic (seeded scenes generated on the host)"}
     value = None
+    bw = None
+    if "bw" in legs:
+        bw = bench_bandwidth()
+        out["hbm_ceiling"] = bw
+        log(f"HBM ceiling (streaming probe): {bw}")
     if "batch" in legs:
         iters = mix[0]["iters"]
         cells = sum((sc["size"] - 2) ** 2 for sc in mix)
@@ -556,7 +648,8 @@ def main():
                 "scenes": len(mix),
                 "interior_cells": cells,
                 "gs_sweeps": iters,
-                "mode": "wavefront (bit-exact vs CPU oracle)",
+                "mode": "wavefront, bit-exact: the CPU leg checks one timed scene per size against the "
+                        "oracle after all frames (config.bit_exact_check)",
                 "batch_tuning": tuning,
             },
             "roofline": {
@@ -566,13 +659,19 @@ def main():
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": profiled_traffic("eqf_frames_cyclic"),
-                "traffic_source": "profiles/eqf_frames_cyclic_batched_mix.md (rocprofv3 --pmc FETCH_SIZE, "
-                                  "WRITE_SIZE, scaled by a 1 GiB copy)",
                 "alg_bytes_per_launch": alg_bytes,
                 "launch_ms": launch_ms,
             },
         })
+        shape = f"{tuning['max_waves']}:{0 if tuning['order'] == 'largest first' else 1}"
+        traffic, ent = profiled_traffic("eqf_frames_cyclic", shape)
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_source"] = (
+            f"{ent['source']} at launch shape {shape} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, scaled by a copy "
+            "of known size)" if ent else f"no rocprofv3 profile at launch shape {shape}")
+        if bw:
+            out["roofline"]["practical_peak"] = bw["copy"]
+            out["roofline"]["frac_of_practical"] = achieved / bw["copy"]
 
     if "rbmix" in legs:
         # the batched mix in red-black mode (<= 1e-5 vs FP64 instead of bit-exact)
@@ -640,10 +739,14 @@ def main():
             log(f"red-black batch vs fp64: {out['batched_mix_redblack']['vs_fp64']}")
         if "rb4096" in CHECKS:
             check_redblack_4096(*CHECKS["rb4096"])
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = len(os.sched_getaffinity(0))  # every core this process may run on
+        if "batch" in CHECKS:
+            out["config"]["bit_exact_check"] = check_batch_exact(*CHECKS["batch"], threads)
+            log(f"batch bit-exact check: {out['config']['bit_exact_check']}")
         rate, wcpu, cframes = cpu_baseline(mix, args.cpu_seconds, threads)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                               "cores_note": "all cores in this process's CPU affinity set",
                                "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) after one untimed "
                                          f"warm-up frame, one scene per thread, {wcpu:.1f} s"}
         log(f"cpu oracle: {rate:.4g} cell-updates/s on {threads} threads ({wcpu:.1f} s)")
