@@ -115,7 +115,12 @@ EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
 // Rows are P + 1 floats apart, so that lane 63's edge read (the strip below)
 // and the other lanes' shared read (the strip above) fall in different banks.
 EQF_HD inline int cyc_lds_row(int P) { return P + 1; }
-EQF_HD inline int cyc_lds_floats(int ns, int P) { return 2 * ns * cyc_lds_row(P) + 16; }
+// The one-pass phases reuse the space for the advect trigger columns (one int
+// per row: 64 per strip).
+EQF_HD inline int cyc_lds_floats(int ns, int P) {
+  const int h = 2 * ns * cyc_lds_row(P) + 16;
+  return h > 64 * ns ? h : 64 * ns;
+}
 
 // Ring offset inside a cyclic field.
 EQF_HD inline int64_t cyc_ring(int P, int ns) { return (int64_t)ns * P * 64; }

@@ -359,6 +359,11 @@ __device__ __forceinline__ int edge_index(int t, int lane, int rl, bool has_prev
   return lane == 0 ? (has_prev ? tu * 64 : min(t, N - 1) + 1) : (has_next ? tl * 64 : S + min(tl, N - 1) + 1);
 }
 
+// The one-pass phases deal blocks of slots to all the waves of the workgroup:
+// item k covers strip k / nb, slots [(k % nb) * BS, +BS) (nb = P / BS), and
+// wave w takes items w, w + W, w + 2W, ...  Adjacent items are adjacent
+// blocks of one strip.  An item loads the slot before its block itself.
+
 // project, first half (fluid.rs:339-349): div and p = 0 on interior cells,
 // then set_boundaries(Passive) on both (fluid.rs:351-352).
 __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__ u, gcf* __restrict__ v,
@@ -369,7 +374,10 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
   gcf* uring = u + ns * PS;
   gcf* vring = v + ns * PS;
   const float sf = sc.sf;
-  for (int s = uni(threadIdx.x >> 6); s < ns; s += (blockDim.x >> 6)) {
+  const int nb = P / 16;
+  for (int item = uni(threadIdx.x >> 6); item < ns * nb; item += (blockDim.x >> 6)) {
+    const int s = item / nb;
+    const int tb = (item - s * nb) * 16;
     const int j = 64 * s + lane + 1;
     const bool row_ok = j <= N;
     const int jc = min(j, N);
@@ -384,40 +392,37 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
     gf* ps = p + s * PS + lane;
     const float uL = uring[2 * S + jc], uR = uring[3 * S + jc];
     const int lane_nm1 = (lane + N - 1) % P;
-    float um1 = us[(P - 1) * 64], vm1 = vs[(P - 1) * 64];  // slot before the block
-    for (int tb = 0; tb < P; tb += 16) {
-      float ub[17], vb[17], e[16];
-      const int tn = tb + 16 == P ? 0 : tb + 16;
+    const int tp = tb == 0 ? P - 1 : tb - 1;  // the slot before the block
+    float ub[17], vb[17], e[16];
+    const int tn = tb + 16 == P ? 0 : tb + 16;
+    const float um1 = us[tp * 64], vm1 = vs[tp * 64];
 #pragma unroll
-      for (int k = 0; k < 17; ++k) {
-        const int t = k < 16 ? tb + k : tn;
-        ub[k] = us[t * 64];
-        vb[k] = vs[t * 64];
+    for (int k = 0; k < 17; ++k) {
+      const int t = k < 16 ? tb + k : tn;
+      ub[k] = us[t * 64];
+      vb[k] = vs[t * 64];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) e[k] = eb[edge_index(tb + k, lane, rl, has_prev, has_next, N, P, S)];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int t = tb + k;
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;
+      const float um = k == 0 ? um1 : ub[k - 1];
+      const float vm = k == 0 ? vm1 : vb[k - 1];
+      // neighbours (fluid.rs:342-344)
+      const float uplus = t == lane_nm1 ? uR : ub[k + 1];
+      const float uminus = t == lane ? uL : um;
+      // below: lane+1's value; the last lane gets e, its own (lane 63) or
+      // lane rl+1's (which loaded the value below the last row)
+      const float vplus = from_next_lane(lane == rl + 1 ? e[k] : vb[k + 1], e[k]);
+      const float vminus = from_prev_lane(vm, e[k]);  // lane 0: the value above the strip
+      const float d = -0.5f * (uplus - uminus + vplus - vminus) / sf;
+      if (row_ok && c0 < N) {
+        ds[t * 64] = d;
+        ps[t * 64] = 0.0f;
       }
-#pragma unroll
-      for (int k = 0; k < 16; ++k) e[k] = eb[edge_index(tb + k, lane, rl, has_prev, has_next, N, P, S)];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const int t = tb + k;
-        int c0 = t - lane;
-        c0 += c0 < 0 ? P : 0;
-        const float um = k == 0 ? um1 : ub[k - 1];
-        const float vm = k == 0 ? vm1 : vb[k - 1];
-        // neighbours (fluid.rs:342-344)
-        const float uplus = t == lane_nm1 ? uR : ub[k + 1];
-        const float uminus = t == lane ? uL : um;
-        // below: lane+1's value; the last lane gets e, its own (lane 63) or
-        // lane rl+1's (which loaded the value below the last row)
-        const float vplus = from_next_lane(lane == rl + 1 ? e[k] : vb[k + 1], e[k]);
-        const float vminus = from_prev_lane(vm, e[k]);  // lane 0: the value above the strip
-        const float d = -0.5f * (uplus - uminus + vplus - vminus) / sf;
-        if (row_ok && c0 < N) {
-          ds[t * 64] = d;
-          ps[t * 64] = 0.0f;
-        }
-      }
-      um1 = ub[15];
-      vm1 = vb[15];
     }
   }
   __syncthreads();
@@ -432,7 +437,10 @@ __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__
   const int PS = P * 64;
   gcf* pring = p + ns * PS;
   const float sf = sc.sf;
-  for (int s = uni(threadIdx.x >> 6); s < ns; s += (blockDim.x >> 6)) {
+  const int nb = P / 8;  // 8 slots keep an item within 128 registers
+  for (int item = uni(threadIdx.x >> 6); item < ns * nb; item += (blockDim.x >> 6)) {
+    const int s = item / nb;
+    const int tb = (item - s * nb) * 8;
     const int j = 64 * s + lane + 1;
     const bool row_ok = j <= N;
     const int jc = min(j, N);
@@ -444,35 +452,33 @@ __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__
     gf* vs = v + s * PS + lane;
     const float pL = pring[2 * S + jc], pR = pring[3 * S + jc];
     const int lane_nm1 = (lane + N - 1) % P;
-    float pm1 = ps[(P - 1) * 64];
-    for (int tb = 0; tb < P; tb += 8) {  // 8 slots keep the block within 128 registers
-      float pb[9], e[8], ub[8], vb[8];
-      const int tn = tb + 8 == P ? 0 : tb + 8;
+    const int tp = tb == 0 ? P - 1 : tb - 1;
+    float pb[9], e[8], ub[8], vb[8];
+    const int tn = tb + 8 == P ? 0 : tb + 8;
+    const float pm1 = ps[tp * 64];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) pb[k] = ps[(k < 8 ? tb + k : tn) * 64];
+    for (int k = 0; k < 9; ++k) pb[k] = ps[(k < 8 ? tb + k : tn) * 64];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int t = tb + k;
-        e[k] = eb[edge_index(t, lane, rl, has_prev, has_next, N, P, S)];
-        ub[k] = us[t * 64];
-        vb[k] = vs[t * 64];
+    for (int k = 0; k < 8; ++k) {
+      const int t = tb + k;
+      e[k] = eb[edge_index(t, lane, rl, has_prev, has_next, N, P, S)];
+      ub[k] = us[t * 64];
+      vb[k] = vs[t * 64];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int t = tb + k;
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;
+      const float pm = k == 0 ? pm1 : pb[k - 1];
+      const float pplus = t == lane_nm1 ? pR : pb[k + 1];
+      const float pminus = t == lane ? pL : pm;
+      const float pdn = from_next_lane(lane == rl + 1 ? e[k] : pb[k + 1], e[k]);
+      const float pup = from_prev_lane(pm, e[k]);
+      if (row_ok && c0 < N) {
+        us[t * 64] = ub[k] - 0.5f * (pplus - pminus) * sf;
+        vs[t * 64] = vb[k] - 0.5f * (pdn - pup) * sf;
       }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int t = tb + k;
-        int c0 = t - lane;
-        c0 += c0 < 0 ? P : 0;
-        const float pm = k == 0 ? pm1 : pb[k - 1];
-        const float pplus = t == lane_nm1 ? pR : pb[k + 1];
-        const float pminus = t == lane ? pL : pm;
-        const float pdn = from_next_lane(lane == rl + 1 ? e[k] : pb[k + 1], e[k]);
-        const float pup = from_prev_lane(pm, e[k]);
-        if (row_ok && c0 < N) {
-          us[t * 64] = ub[k] - 0.5f * (pplus - pminus) * sf;
-          vs[t * 64] = vb[k] - 0.5f * (pdn - pup) * sf;
-        }
-      }
-      pm1 = pb[7];
     }
   }
   __syncthreads();
@@ -518,22 +524,24 @@ __device__ __forceinline__ float gather(gcf* d0, const Trace& r, int S, int P, i
 // `break` at fluid.rs:420-423 is reproduced per row: cells left of the first
 // trigger column get the bilinear value, the trigger cell copies its left
 // neighbour (the ring cell when it is column 1), cells to the right keep the
-// buffer's previous content.
-__device__ __forceinline__ void advect_phase(const SceneDesc& sc, int mode0, gf* __restrict__ d0out,
-                                             gcf* __restrict__ d0in, int mode1, gf* __restrict__ d1out,
-                                             gcf* __restrict__ d1in, gcf* __restrict__ u, gcf* __restrict__ v) {
+// buffer's previous content.  Pass A (one wave per strip) finds each row's
+// first trigger column and leaves it in LDS (`trig_lds`, one int per row);
+// pass B deals blocks of 8 slots to all waves.
+__device__ __forceinline__ void advect_phase(const SceneDesc& sc, lf* __restrict__ trig_lds, int mode0,
+                                             gf* __restrict__ d0out, gcf* __restrict__ d0in, int mode1,
+                                             gf* __restrict__ d1out, gcf* __restrict__ d1in, gcf* __restrict__ u,
+                                             gcf* __restrict__ v) {
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
   const int PS = P * 64;
   const float dt0 = sc.dt0, sf = sc.sf, sfm1 = sc.sfm1;
   const bool two = d1out != nullptr;
+  // pass A: first trigger column of every row
   for (int s = uni(threadIdx.x >> 6); s < ns; s += (blockDim.x >> 6)) {
     const int j = 64 * s + lane + 1;
-    const bool row_ok = j <= N;
     const int jc = min(j, N);
     gcf* us = u + s * PS + lane;
     gcf* vs = v + s * PS + lane;
-    // pass A: first trigger column of this row
     int trig = N + 1;
     for (int tb = 0; tb < P; tb += 16) {
       float ub[16], vb[16];
@@ -550,41 +558,51 @@ __device__ __forceinline__ void advect_phase(const SceneDesc& sc, int mode0, gf*
         if (c0 < N && r.trig && c0 + 1 < trig) trig = c0 + 1;
       }
     }
-    // pass B
-    const int ro = ns * PS;
+    trig_lds[64 * s + lane] = __builtin_bit_cast(float, trig);
+  }
+  __syncthreads();
+  // pass B
+  const int ro = ns * PS;
+  const int nb = P / 8;  // 8 slots: velocity loads, then 32-64 gathers in flight
+  for (int item = uni(threadIdx.x >> 6); item < ns * nb; item += (blockDim.x >> 6)) {
+    const int s = item / nb;
+    const int tb = (item - s * nb) * 8;
+    const int j = 64 * s + lane + 1;
+    const bool row_ok = j <= N;
+    const int jc = min(j, N);
+    const int trig = __builtin_bit_cast(int, trig_lds[64 * s + lane]);
+    gcf* us = u + s * PS + lane;
+    gcf* vs = v + s * PS + lane;
     const float left0 = d0out[ro + 2 * S + jc];
     const float left1 = two ? d1out[ro + 2 * S + jc] : 0.0f;
     gf* o0 = d0out + s * PS + lane;
     gf* o1 = (two ? d1out : d0out) + s * PS + lane;
-    float um1 = us[(P - 1) * 64], vm1 = vs[(P - 1) * 64];
-    for (int tb = 0; tb < P; tb += 8) {  // 8 slots: velocity loads, then 32-64 gathers in flight
-      float ub[8], vb[8];
+    const int tp = tb == 0 ? P - 1 : tb - 1;
+    const float um1 = us[tp * 64], vm1 = vs[tp * 64];
+    float ub[8], vb[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        ub[k] = us[(tb + k) * 64];
-        vb[k] = vs[(tb + k) * 64];
-      }
+    for (int k = 0; k < 8; ++k) {
+      ub[k] = us[(tb + k) * 64];
+      vb[k] = vs[(tb + k) * 64];
+    }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int t = tb + k;
-        int c0 = t - lane;
-        c0 += c0 < 0 ? P : 0;
-        const int c = c0 + 1;
-        // c < trig: this cell; c == trig > 1: the left neighbour's value
-        const bool at = c == trig;
-        const float uu = at ? (k == 0 ? um1 : ub[k - 1]) : ub[k];
-        const float vv = at ? (k == 0 ? vm1 : vb[k - 1]) : vb[k];
-        const Trace tr = back_trace(at ? c - 1 : c, j, uu, vv, dt0, sf, sfm1, S);
-        const float g0 = gather(d0in, tr, S, P, ns);
-        const float g1 = two ? gather(d1in, tr, S, P, ns) : 0.0f;
-        if (row_ok && c <= N && c <= trig) {
-          const bool ring1 = at && c == 1;  // trigger in column 1: copy the ring cell (fluid.rs:421)
-          o0[t * 64] = ring1 ? left0 : g0;
-          if (two) o1[t * 64] = ring1 ? left1 : g1;
-        }
+    for (int k = 0; k < 8; ++k) {
+      const int t = tb + k;
+      int c0 = t - lane;
+      c0 += c0 < 0 ? P : 0;
+      const int c = c0 + 1;
+      // c < trig: this cell; c == trig > 1: the left neighbour's value
+      const bool at = c == trig;
+      const float uu = at ? (k == 0 ? um1 : ub[k - 1]) : ub[k];
+      const float vv = at ? (k == 0 ? vm1 : vb[k - 1]) : vb[k];
+      const Trace tr = back_trace(at ? c - 1 : c, j, uu, vv, dt0, sf, sfm1, S);
+      const float g0 = gather(d0in, tr, S, P, ns);
+      const float g1 = two ? gather(d1in, tr, S, P, ns) : 0.0f;
+      if (row_ok && c <= N && c <= trig) {
+        const bool ring1 = at && c == 1;  // trigger in column 1: copy the ring cell (fluid.rs:421)
+        o0[t * 64] = ring1 ? left0 : g0;
+        if (two) o1[t * 64] = ring1 ? left1 : g1;
       }
-      um1 = ub[7];
-      vm1 = vb[7];
     }
   }
   __syncthreads();
@@ -682,7 +700,7 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
     } else if (kind == PK_GRAD) {
       grad_phase(sc, x, y, z);
     } else if (kind == PK_ADV) {
-      advect_phase(sc, mode, x, y, mode1, z, w, q, r);
+      advect_phase(sc, H, mode, x, y, mode1, z, w, q, r);
     } else {
       copy_phase(sc, x, y);
     }
