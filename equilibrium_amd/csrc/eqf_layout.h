@@ -106,7 +106,24 @@ struct SceneDesc {
 #define EQF_GS_B 16
 #endif
 
+
 EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
+
+// Wavefront lags for blocks of B steps (B divides 16) and prefetch depth D,
+// both multiples of B.  A value handed from one wave to another must be read
+// at least B steps after it was stored (a block barrier lies in between
+// whatever the alignment), and reads are issued D steps before use:
+//  * strip s+1 trails strip s by L >= 64 + B + D steps: its lane 0 reads strip
+//    s's raw last-row value of column c, stored at strip-local step c + 62,
+//    at step c - 1 - D of its own;
+//  * sweep stream j+1 of a strip trails stream j by LAG >= L - 62 + B + D: its
+//    last lane reads the strip below's post-boundary row-0 value of column c,
+//    which stream j of that strip stores at its step c (AdjustRow /
+//    AdjustColumn store a cell's final value one step after computing it),
+//    at step c + 62 - D of its own (the AdjustColumn fix-up is stored a step
+//    earlier).
+EQF_HD constexpr int cyc_strip_lag(int B, int D) { return (64 + B + D + B - 1) / B * B; }
+EQF_HD constexpr int cyc_stream_lag(int B, int D) { return (cyc_strip_lag(B, D) - 62 + B + D + B - 1) / B * B; }
 
 // Exact mode keeps the strips' hand-off rows in LDS: 2*ns rows of P floats
 // (top ring, raw last rows, post-boundary first rows, bottom ring) plus a

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -535,7 +536,8 @@ extern "C" int eqf_download(eqf_scene* s, int field, float* host) {
 }
 
 int eqf_scene_waves(int ns, int P) {
-  const int m = std::max(1, std::min(16 / std::max(ns, 1), P / 64));  // sweep streams per strip
+  const int lag = cyc_stream_lag(EQF_GS_B, EQF_GS_D);
+  const int m = std::max(1, std::min(16 / std::max(ns, 1), P / lag));  // sweep streams per strip
   return std::max(1, std::min(16, ns * m));
 }
 
@@ -555,9 +557,7 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
   const int W = eqf_scene_waves(s->ns, s->P);
   while (nframes > 0) {
     const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
-    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(1), dim3(64 * W),
-                       sizeof(float) * cyc_lds_floats(s->ns, s->P), s->stream, s->d_desc, nf);
-    HIPCHK(hipGetLastError());
+    HIPCHK(eqf_internal_launch(s->d_desc, 1, W, cyc_lds_floats(s->ns, s->P), nf, s->stream));
     nframes -= nf;
   }
   return EQF_OK;

@@ -172,9 +172,17 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 template <int MODE, int D, int B>
 __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict__ H, gf* __restrict__ X,
                                              gcf* __restrict__ X0, float a, float cr, int iters) {
-  static_assert(B == 16 && D < B, "blocks of 16 steps (one selector word), deeper than the prefetch");
-  constexpr int L = (64 + B + D + B - 1) / B * B;  // strip lag, a multiple of B
-  constexpr int LAG = 64;                          // sweep lag between streams of a strip
+  static_assert((B == 16 || B == 8) && D < B, "blocks of 8 or 16 steps (within one selector word), deeper "
+                                             "than the prefetch");
+  // the prefetch registers form a ring indexed by the step within the block
+  // modulo D, which carries over from block to block only if D divides B
+  static_assert(B % D == 0, "prefetch depth must divide the block length");
+  constexpr int L = cyc_strip_lag(B, D);    // strip lag, a multiple of B
+  constexpr int LAG = cyc_stream_lag(B, D);  // sweep lag between streams of a strip
+  // within a strip, stream j+1 loads a slot stream j stored B + D + 2 steps
+  // earlier at least (one step of delayed store, one of prefetch offset)
+  static_assert(LAG >= B + D + 2, "stream lag too short for the prefetch");
+  static_assert(L >= 64 + B + D && LAG >= L - 62 + B + D, "hand-off lags too short");
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
   const int wave = uni(threadIdx.x >> 6);
@@ -727,3 +735,4 @@ __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* 
 
 template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>(const eqf::SceneDesc* __restrict__, int);
 template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>(const eqf::SceneDesc* __restrict__, int);
+
