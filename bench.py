@@ -201,7 +201,7 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
     cells = sum((sc["size"] - 2) ** 2 for sc in scs)
     res = {"value": cells * steps / wall, "unit": "cell-updates/s", "ms_per_frame": wall * 1e3 / steps,
            "launch_ms_per_frame": ev_ms / steps, "band_waves_rows": list(tune.TILES[tile]),
-           "sweeps_per_launch": tsw, "roofline_alg_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
+           "sweeps_per_launch": tsw, "streaming_model_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
                HBM_PEAK_GBS * 1e9)}
     if check_frames > 0:
         # GPU fields after each of the first frames of one scene per size,
@@ -389,7 +389,20 @@ def bench_redblack_4096(steps, check_frames, N=4096, tiling=None):
     res = {"value": rate, "unit": "cell-updates/s", "ms_per_frame": wall * 1e3 / steps,
            "frames_timed": steps, "band_waves_rows": list(tune.TILES[tile]), "sweeps_per_launch": tsw,
            "gs_sweeps": sc["iters"], "alg_bytes_per_cell_update": b_alg(sc["iters"]),
-           "roofline_frac": rate * b_alg(sc["iters"]) / (HBM_PEAK_GBS * 1e9)}
+           # streaming-model rate: k_rb_band keeps its region in registers for
+           # several sweeps, so this can exceed 1 and is no HBM fraction
+           "streaming_model_frac": rate * b_alg(sc["iters"]) / (HBM_PEAK_GBS * 1e9)}
+    shape = f"{tile}:{tsw}"
+    traffic, ent = profiled_traffic("rb_frame_n4096", shape)
+    if traffic and N == 4096:
+        gbs = traffic / (wall / steps) / 1e9
+        res["measured_hbm"] = {"bytes_per_frame": traffic, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                               "source": f"{ent['source']} at band:sweeps {shape} (rocprofv3 --pmc FETCH_SIZE, "
+                                         "WRITE_SIZE per kernel x launches per frame)"}
+    else:
+        res["measured_hbm"] = None
+        res["measured_hbm_note"] = f"no rocprofv3 profile at band:sweeps {shape}"
     h.close()
     if check_frames > 0:
         # GPU fields after each of the first frames and the last projection's

@@ -112,6 +112,7 @@ def lib():
             "eqf_last_error": (ctypes.c_char_p, []),
             "eqf_ctx_create": (I, [I, PP]),
             "eqf_ctx_destroy": (I, [P]),
+            "eqf_ctx_device_key": (I, [P, ctypes.c_char_p, ctypes.c_size_t]),
             "eqf_scene_create": (I, [P, ctypes.POINTER(Params), PP]),
             "eqf_scene_clone": (I, [P, PP]),
             "eqf_scene_destroy": (I, [P]),
@@ -194,6 +195,12 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().eqf_ctx_create(device, ctypes.byref(h)))
         self.h = h
+
+    def device_key(self):
+        """'<device name>/<gfx arch>/<CU count>' of the context's GPU."""
+        buf = ctypes.create_string_buffer(512)
+        check(lib().eqf_ctx_device_key(self.h, buf, len(buf)))
+        return buf.value.decode(errors="replace")
 
     def close(self):
         if self.h:

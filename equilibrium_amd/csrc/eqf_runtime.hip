@@ -196,6 +196,18 @@ extern "C" int eqf_ctx_destroy(eqf_ctx* ctx) {
   return EQF_OK;
 }
 
+extern "C" int eqf_ctx_device_key(const eqf_ctx* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || len == 0) return fail(EQF_EINVAL, "ctx or buf is NULL, or len is 0");
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  const int n = snprintf(buf, len, "%s/%s/%dCU", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  if (n < 0 || (size_t)n >= len) {
+    buf[0] = 0;
+    return fail(EQF_EINVAL, "buffer too small for the device key");
+  }
+  return EQF_OK;
+}
+
 hipError_t eqf_scene_malloc(eqf_scene* s, void** p, size_t bytes) {
   if (s->pool) return hipMallocFromPoolAsync(p, bytes, s->pool, s->pool_stream);
   return hipMalloc(p, bytes);

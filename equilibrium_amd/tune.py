@@ -10,7 +10,8 @@
 Each times frames per configuration on the GPU (on clones, so the tuned
 scenes are not advanced) and keeps the fastest; results are cached in a JSON
 file (EQF_TUNE_CACHE, default equilibrium_amd/tuning_cache.json) keyed by the
-workload and the device name, so later runs reuse them.
+workload and the device (name, gfx architecture and CU count, as the library
+reports them for the scenes' context), so later runs reuse them.
 """
 import hashlib
 import json
@@ -26,14 +27,8 @@ CACHE = os.environ.get("EQF_TUNE_CACHE", os.path.join(os.path.dirname(os.path.ab
                                                        "tuning_cache.json"))
 
 
-def _device_name():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            return torch.cuda.get_device_name(0)
-    except Exception:
-        pass
-    return "unknown"
+def _device_key(ctx):
+    return ctx.device_key()
 
 
 def _load():
@@ -57,7 +52,7 @@ def tune_redblack(scene, frames=1, use_cache=True):
     returns (tile, sweeps, timings).  The scene is advanced by the trial
     frames; callers that need an untouched state tune a clone."""
     p = scene.params
-    key = f"rbband:{p.size}:{p.gs_iters}:{p.pressure_iters}:{_device_name()}"
+    key = f"rbband:{p.size}:{p.gs_iters}:{p.pressure_iters}:{_device_key(scene.ctx)}"
     db = _load() if use_cache else {}
     if key in db:
         tile, tsw = db[key]["best"]
@@ -87,7 +82,7 @@ BATCH_ORDERS = (_abi.BATCH_LARGEST_FIRST, _abi.BATCH_GIVEN_ORDER)
 
 def _batch_key(scenes):
     sig = ",".join(f"{s.params.size}:{s.params.gs_iters}:{s.params.pressure_iters}" for s in scenes)
-    return f"batch:{len(scenes)}:{hashlib.sha1(sig.encode()).hexdigest()[:16]}:{_device_name()}"
+    return f"batch:{len(scenes)}:{hashlib.sha1(sig.encode()).hexdigest()[:16]}:{_device_key(scenes[0].ctx)}"
 
 
 def tune_batch(scenes, frames=2, use_cache=True):

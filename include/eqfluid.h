@@ -82,6 +82,10 @@ const char *eqf_last_error(void);
 /* One context per GPU: owns the device selection. */
 int eqf_ctx_create(int device, eqf_ctx **out);
 int eqf_ctx_destroy(eqf_ctx *ctx);
+/* No counterpart in the reference (host plumbing for tuning caches): writes
+ * "<device name>/<gfx arch>/<CU count>" of the context's GPU, NUL-terminated,
+ * into buf; EQF_EINVAL if it does not fit in len bytes. */
+int eqf_ctx_device_key(const eqf_ctx *ctx, char *buf, size_t len);
 
 /* Fluid::new / Fluid::default (fluid.rs:83-110): allocates the six fields and
  * the mask on the device and writes the reference's initial state. */

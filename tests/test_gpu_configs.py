@@ -15,6 +15,12 @@
   the f64 red-black oracle (norm-wise relative error of density and velocity
   at most 1e-5 per frame) and the pressure Poisson residual of the last
   projection beside the f64 oracle's.
+* Red-black accuracy on the mix: the first scene of each size of the
+  512-scene mix in red-black mode, 4 frames, each frame's norm-wise relative
+  error against the f64 red-black oracle at most max(1e-5, 2x the error of
+  the reference's own f32 step against its f64 version on the same scene).
+  The scenes are chaotic: both errors grow ~5x per frame and reach 0.1-0.6
+  by frame 20, so the bar is per frame and relative to the reference's f32.
 * Serving: one second of Poisson arrivals at 200 requests/s with N
   log-uniform in [64, 2048] (exact mode up to N=1024, red-black above), frames
   capped at 6; every result equals its solo run bit for bit.
@@ -164,6 +170,51 @@ def test_redblack_4096_bit_exact_and_fp64_error():
     mx64, rr64, rd64 = oracle.poisson_residual(st64.f["vx0"], st64.f["vy0"], S)
     print(f"[poisson] gpu rms/div rms {rr / rd:.6f}, f64 {rr64 / rd64:.6f}, worst rel {worst:.3g}", flush=True)
     assert abs(rr / rd - rr64 / rd64) <= 1e-3 * (rr64 / rd64)
+
+
+def test_redblack_mix_accuracy_against_fp64_beside_reference_f32():
+    mix = scenes.batched_mix(frames=4)
+    picks = [[sc for sc in mix if sc["size"] - 2 == N][0] for N in (64, 128, 256, 512)]
+    nf = 4
+    _case(f"red-black mix accuracy: N in 64..512, {nf} frames vs f64 beside the reference's f32")
+    got = []
+    for sc in picks:
+        h = gpu_scene(sc, _abi.GS_REDBLACK)
+        frames = []
+        for _ in range(nf):
+            h.step(1)
+            h.sync()
+            frames.append({k: h.download(fid) for k, fid in (("density", _abi.DENSITY), ("vx", _abi.VX),
+                                                             ("vy", _abi.VY))})
+        h.close()
+        got.append(frames)
+
+    def one(i):
+        sc = picks[i]
+        sts = []
+        for dtype, order in ((np.float64, oracle.RB), (np.float32, oracle.LEX), (np.float64, oracle.LEX)):
+            st = oracle_state(sc).astype(dtype)
+            sts.append((st, order))
+        rows = []
+        for fields in got[i]:
+            for st, order in sts:
+                st.run(sc["dt"], sc["diffusion"], sc["viscosity"], sc["iters"], 1, order=order,
+                       src_d=sc["src_d"], src_u=sc["src_u"], src_v=sc["src_v"])
+            rb64, lex32, lex64 = (st for st, _ in sts)
+            ours = max(_rel(fields[k], rb64.f[k]) for k in fields)
+            ref = max(_rel(lex32.f[k], lex64.f[k]) for k in fields)
+            rows.append((ours, ref))
+        return rows
+
+    with ThreadPoolExecutor(max_workers=len(picks)) as ex:
+        res = list(ex.map(one, range(len(picks))))
+    bad = []
+    for sc, rows in zip(picks, res):
+        N = sc["size"] - 2
+        print(f"[fp64] N={N}: ours {[f'{o:.3g}' for o, _ in rows]} reference f32 {[f'{r:.3g}' for _, r in rows]}",
+              flush=True)
+        bad += [(N, f, o, r) for f, (o, r) in enumerate(rows) if o > max(1e-5, 2.0 * r)]
+    assert not bad, bad
 
 
 def test_serving_poisson_arrivals_equal_solo_runs():
