@@ -61,7 +61,7 @@ def profiled_traffic(kernel, shape=None):
         ent = None
     if ent is None:
         return None, None
-    return ent.get("traffic_bytes_per_launch"), ent
+    return ent.get("traffic_bytes_per_launch", ent.get("traffic_bytes_per_frame")), ent
 
 
 def bench_bandwidth(gib=8, blocks_per_cu=(4, 8), reps=5):

@@ -4,12 +4,15 @@
 // first and the odd cells second, then applies set_boundaries, so all cells
 // of one colour are independent and the whole grid is updated in parallel by
 // many workgroups.  Sweeps are temporally blocked in registers (k_rb_band):
-// a workgroup loads a region (its output tile plus a halo of 3T cells), runs
-// T full sweeps (red, black, set_boundaries) on it, and writes the tile back,
-// so x and x0 cross HBM once per T sweeps instead of once per sweep.  Each
-// full sweep shrinks the valid region by 3 cells: red reads black neighbours
-// (1), black reads the new red neighbours (2), and set_boundaries copies a
-// wall neighbour (3).  A pass reads one buffer and writes another (ping-pong).
+// a workgroup loads a region (its output tile plus a halo of 3T cells on the
+// sides that face other tiles), runs T full sweeps (red, black,
+// set_boundaries) on it, and writes the tile back, so x and x0 cross HBM once
+// per T sweeps instead of once per sweep.  Each full sweep shrinks the valid
+// region by 3 cells on every side with cells beyond it: red reads black
+// neighbours (1), black reads the new red neighbours (2), and set_boundaries
+// copies a wall neighbour (3).  A side on the grid's edge has nothing beyond
+// it, so the first and last tiles of a row or column of tiles reach the edge
+// with no halo there.  A pass reads one buffer and writes another (ping-pong).
 //
 // The other phases (divergence, gradient, advection with its row `break`,
 // set_boundaries lists, copy) are grid-wide kernels.  One frame is captured
@@ -129,6 +132,13 @@ __device__ __forceinline__ int find_scene(const I* __restrict__ start, int n, I 
   }
   return lo;
 }
+// Tiles along one dimension of S cells for regions of W cells and output
+// tiles of TX: region b starts at b*TX; the first needs no halo before it
+// (the grid's edge), the last must reach the far edge.
+__host__ __device__ __forceinline__ int band_span(int S, int W, int TX) {
+  return S <= W ? 1 : (S - W + TX - 1) / TX + 1;
+}
+
 // lin_solve coefficients (a, 1/c) of a scene: velocity diffusion, density
 // diffusion, pressure (fluid.rs:286-294, 311, 356).
 __device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, float& cr) {
@@ -146,8 +156,11 @@ __device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, fl
 // Red cells have (x + y) even; the region origin is chosen so that row i of
 // every band has its red cell in column (i & 1) of the lane.  Every full
 // sweep shrinks the valid region by 3 cells (red reads black, black reads the
-// new red, set_boundaries reads a neighbour), so the launch stores the
-// (128 - 6T) x (NW*R - 6T) interior of its region; the region reads one
+// new red, set_boundaries reads a neighbour) on each side that has cells
+// beyond it, so the launch stores the (128 - 6T) x (NW*R - 6T) interior of its
+// region, extended to the grid's edge on the sides where the region reaches
+// it (tile b of a dimension: region origin b*TX, so the first region starts
+// at the edge and the last one passes the far edge); the region reads one
 // buffer and the tile is written to the other (ping-pong).  Corners are set
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
@@ -176,7 +189,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
   const RbScene& sc = scs[js];
   const SceneDesc& d = sc.d;
   const int S = d.S, N = d.N;
-  const int tiles_x = (S + TX - 1) / TX;
+  const int tiles_x = band_span(S, W, TX), tiles_y = band_span(S, RH, TY);
   const int bx = tl % tiles_x, by = tl / tiles_x;
   // Global-memory views: a row's base address is wave-uniform (scalar
   // registers), a lane's column a 32-bit offset.
@@ -186,8 +199,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
   GAS const uint8_t* __restrict__ code = (GAS const uint8_t*)d.code;
   float a, cr;
   coefs(d, coef, a, cr);
-  const int gx0 = bx * TX - H;  // region origin; gx0 + gy0 is even (TX, TY, R even)
-  const int gyb = by * TY - H;
+  const int gx0 = bx * TX;  // region origin; gx0 + gy0 is even (TX, TY, R even)
+  const int gyb = by * TY;
   const int gy0 = gyb + w * R;          // this wave's band
   const int gxa = gx0 + 2 * lane;
   // this lane's two columns, clamped into the grid (cells outside it are
@@ -335,17 +348,22 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
   }
 
   // (columns recomputed here rather than kept live through the sweeps)
-  const int gxl = bx * TX - H + 2 * (int)(threadIdx.x & 63);
+  // Output tile: [gx0 + H, gx0 + H + TX) x [gyb + H, gyb + H + TY), from the
+  // grid's edge for the first tile of a dimension and to its far edge for
+  // the last one.
+  const int gxl = bx * TX + 2 * (int)(threadIdx.x & 63);
+  const int xlo = bx == 0 ? 0 : bx * TX + H, xhi = bx == tiles_x - 1 ? S : bx * TX + H + TX;
+  const int ylo = by == 0 ? 0 : gyb + H, yhi = by == tiles_y - 1 ? S : gyb + H + TY;
   bool cout[2];  // this lane's columns lie in the output tile
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     const int gx = gxl + b;
-    cout[b] = gx >= gx0 + H && gx < gx0 + H + TX && gx < S;
+    cout[b] = gx >= xlo && gx < xhi;
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int gy = gy0 + i;
-    if (gy < gyb + H || gy >= gyb + H + TY || gy >= S) continue;  // wave-uniform
+    if (gy < ylo || gy >= yhi) continue;  // wave-uniform
     GAS char* dp = (GAS char*)(dst + (int64_t)gy * S);
 #pragma unroll
     for (int b = 0; b < 2; ++b)
@@ -680,7 +698,7 @@ static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}, {8, 18}, {8, 14}};
 static int band_tiles(int S, int tile, int T) {
   const BandCfg b = kBands[tile];
   const int TX = 128 - 6 * T, TY = b.nw * b.r - 6 * T;
-  return ((S + TX - 1) / TX) * ((S + TY - 1) / TY);
+  return rb::band_span(S, 128, TX) * rb::band_span(S, b.nw * b.r, TY);
 }
 
 static uint64_t group_key(const RbGroup& g) {

@@ -22,8 +22,10 @@ def _case(label):
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("tsw", [1, 2, 4, 5, 6, 8])
-def test_redblack_bit_exact_all_tilings(tile, tsw):
-    N = 150
+@pytest.mark.parametrize("N", [150, 330])
+def test_redblack_bit_exact_all_tilings(tile, tsw, N):
+    """N=150: two tiles per dimension (one at each edge); N=330: three or
+    more in most settings, so interior tiles with halos on both sides too."""
     _case(f"rb tile={tile} T={tsw} N={N}")
     sc = scenes.stam_scene(N, scenes.derive_seed(20, N), iters=7, frames=3, n_rects=4,
                            diffusion=1e-4, viscosity=1e-3)
