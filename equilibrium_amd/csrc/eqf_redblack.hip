@@ -840,9 +840,10 @@ static void g_copy(hipStream_t st, const RbGroup& g, int dst, int src) {
 
 // lin_solve in red-black order on every scene of the group (one iteration
 // count): ceil(iters/T) tile passes ping-ponging between x and the scratch
-// field; an odd pass count ends with a copy back; then the corners of the
-// last set_boundaries (fluid.rs:265-271).
-static void g_lin_solve(hipStream_t st, const RbGroup& g, int mode, int x, int x0, int coef, int iters) {
+// field; an odd pass count ends with a copy back; then, if `corners`, the
+// corners of the last set_boundaries (fluid.rs:265-271).
+static void g_lin_solve(hipStream_t st, const RbGroup& g, int mode, int x, int x0, int coef, int iters,
+                        bool corners) {
   if (iters <= 0) return;
   int a_ = x, b_ = rb::ROLE_TMP;
   for (int done = 0; done < iters;) {
@@ -854,50 +855,64 @@ static void g_lin_solve(hipStream_t st, const RbGroup& g, int mode, int x, int x
     done += nsw;
   }
   if (a_ != x) g_copy(st, g, x, a_);
-  hipLaunchKernelGGL(rb::k_corners, dim3((unsigned)((2 * g.n + 255) / 256)), dim3(256), 0, st, g.d_scs, g.n, x,
-                     rb::ROLE_NONE);
+  if (corners)
+    hipLaunchKernelGGL(rb::k_corners, dim3((unsigned)((2 * g.n + 255) / 256)), dim3(256), 0, st, g.d_scs, g.n, x,
+                       rb::ROLE_NONE);
 }
 
-static void g_bnd(hipStream_t st, const RbGroup& g, int mode0, int x0, int mode1, int x1) {
+// set_boundaries lists of one or two fields, then the corners of the fields
+// c0 and c1 (ROLE_NONE: none).
+static void g_bnd(hipStream_t st, const RbGroup& g, int mode0, int x0, int mode1, int x1, int c0, int c1) {
   const int* tab = x1 == rb::ROLE_NONE ? g.d_bp : mode0 == M_PASSIVE ? g.d_bpp : g.d_brc;
   const int total = x1 == rb::ROLE_NONE ? g.bp : mode0 == M_PASSIVE ? g.bpp : g.brc;
   if (total > 0)
     hipLaunchKernelGGL(rb::k_bnd, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g.d_scs, tab, g.n, mode0,
                        x0, mode1, x1);
-  hipLaunchKernelGGL(rb::k_corners, dim3((unsigned)((2 * g.n + 255) / 256)), dim3(256), 0, st, g.d_scs, g.n, x0, x1);
+  if (c0 != rb::ROLE_NONE || c1 != rb::ROLE_NONE)
+    hipLaunchKernelGGL(rb::k_corners, dim3((unsigned)((2 * g.n + 255) / 256)), dim3(256), 0, st, g.d_scs, g.n, c0,
+                       c1);
 }
 
 // One frame (fluid.rs:437-524) of every scene of the group.  All scenes share
 // the iteration counts, so the sequence of launches is the same for all.
+//
+// Corners (fluid.rs:265-271) are read only by advect's bilinear gathers and
+// are part of the frame's final state; no sweep, divergence or gradient reads
+// them.  So a field's corners are computed only where a gather or the end of
+// the frame reads them before the field's next set_boundaries rewrites them:
+// U0 and V0 after the first gradient (advected from), V0 after the second
+// divergence, U0 after the second pressure solve (or after that divergence
+// when the solve has no sweeps), U and V after the second gradient, the
+// density scratch after its diffusion, the density after its advection.
 static void g_frame(hipStream_t st, const RbGroup& g) {
   using rb::ROLE_NONE;
   const dim3 cells((unsigned)g.iblocks);  // interior cell blocks
   if (g.any_src)
     hipLaunchKernelGGL(rb::k_sources, dim3((unsigned)g.cblocks), dim3(256), 0, st, g.d_scs, g.d_cmap);
-  g_lin_solve(st, g, M_ROW, F_U0, F_U, 0, g.diff_iters);  // fluid.rs:438
-  g_lin_solve(st, g, M_COL, F_V0, F_V, 0, g.diff_iters);  // fluid.rs:448
+  g_lin_solve(st, g, M_ROW, F_U0, F_U, 0, g.diff_iters, false);  // fluid.rs:438
+  g_lin_solve(st, g, M_COL, F_V0, F_V, 0, g.diff_iters, false);  // fluid.rs:448
   // project(U0, V0, p=U, div=V), fluid.rs:459
   hipLaunchKernelGGL(rb::k_div, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0, F_V, F_U);
-  g_bnd(st, g, M_PASSIVE, F_V, M_PASSIVE, F_U);
-  g_lin_solve(st, g, M_PASSIVE, F_U, F_V, 2, g.press_iters);
+  g_bnd(st, g, M_PASSIVE, F_V, M_PASSIVE, F_U, ROLE_NONE, ROLE_NONE);
+  g_lin_solve(st, g, M_PASSIVE, F_U, F_V, 2, g.press_iters, false);
   hipLaunchKernelGGL(rb::k_grad, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0, F_U);
-  g_bnd(st, g, M_ROW, F_U0, M_COL, F_V0);
+  g_bnd(st, g, M_ROW, F_U0, M_COL, F_V0, F_U0, F_V0);
   // advect U from U0 and V from V0 with (U0, V0), fluid.rs:469-489
   hipLaunchKernelGGL(rb::k_adv_trig, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U0, F_V0);
   hipLaunchKernelGGL(rb::k_adv, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_U0, F_V, F_V0, F_U0, F_V0);
-  g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
+  g_bnd(st, g, M_ROW, F_U, M_COL, F_V, ROLE_NONE, ROLE_NONE);
   // project(U, V, p=U0, div=V0), fluid.rs:491
   hipLaunchKernelGGL(rb::k_div, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_V, F_V0, F_U0);
-  g_bnd(st, g, M_PASSIVE, F_V0, M_PASSIVE, F_U0);
-  g_lin_solve(st, g, M_PASSIVE, F_U0, F_V0, 2, g.press_iters);
+  g_bnd(st, g, M_PASSIVE, F_V0, M_PASSIVE, F_U0, F_V0, g.press_iters > 0 ? ROLE_NONE : F_U0);
+  g_lin_solve(st, g, M_PASSIVE, F_U0, F_V0, 2, g.press_iters, true);
   hipLaunchKernelGGL(rb::k_grad, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_V, F_U0);
-  g_bnd(st, g, M_ROW, F_U, M_COL, F_V);
+  g_bnd(st, g, M_ROW, F_U, M_COL, F_V, F_U, F_V);
   // density, fluid.rs:501-523
-  g_lin_solve(st, g, M_PASSIVE, F_SC, F_D, 1, g.diff_iters);
+  g_lin_solve(st, g, M_PASSIVE, F_SC, F_D, 1, g.diff_iters, true);
   hipLaunchKernelGGL(rb::k_adv_trig, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_U, F_V);
   hipLaunchKernelGGL(rb::k_adv, cells, dim3(256), 0, st, g.d_scs, g.d_imap, F_D, F_SC, ROLE_NONE, ROLE_NONE,
                      F_U, F_V);
-  g_bnd(st, g, M_PASSIVE, F_D, M_PASSIVE, ROLE_NONE);
+  g_bnd(st, g, M_PASSIVE, F_D, M_PASSIVE, ROLE_NONE, F_D, ROLE_NONE);
   g_copy(st, g, F_SC, F_D);
 }
 
