@@ -203,6 +203,15 @@ def bench_batch_redblack(scs, steps, warmup, check_frames=2, band=None):
            "launch_ms_per_frame": ev_ms / steps, "band_waves_rows": list(tune.TILES[tile]),
            "sweeps_per_launch": tsw, "streaming_model_frac": cells * steps / wall * b_alg(scs[0]["iters"]) / (
                HBM_PEAK_GBS * 1e9)}
+    traffic, ent = profiled_traffic("rb_frame_mix", f"{tile}:{tsw}")
+    if traffic and len(scs) == 512:
+        gbs = traffic / (wall / steps) / 1e9
+        res["measured_hbm"] = {"bytes_per_frame": traffic, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": gbs / HBM_PEAK_GBS,
+                               "source": f"{ent['source']} at band:sweeps {tile}:{tsw} (rocprofv3 --pmc FETCH_SIZE, "
+                                         "WRITE_SIZE per kernel x launches per frame)"}
+    else:
+        res["measured_hbm"] = None
     if check_frames > 0:
         # GPU fields after each of the first frames of one scene per size,
         # checked against the f64 oracle in the CPU leg
