@@ -109,8 +109,8 @@ struct SceneDesc {
 
 EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
 
-// Wavefront lags for blocks of B steps (B divides 16) and prefetch depth D,
-// both multiples of B.  A value handed from one wave to another must be read
+// Wavefront lags for blocks of B steps (B divides 16) and prefetch depth
+// D <= B, both multiples of B.  A value handed from one wave to another must be read
 // at least B steps after it was stored (a block barrier lies in between
 // whatever the alignment), and reads are issued D steps before use:
 //  * strip s+1 trails strip s by L >= 64 + B + D steps: its lane 0 reads strip

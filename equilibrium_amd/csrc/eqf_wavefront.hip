@@ -172,8 +172,8 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 template <int MODE, int D, int B>
 __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict__ H, gf* __restrict__ X,
                                              gcf* __restrict__ X0, float a, float cr, int iters) {
-  static_assert((B == 16 || B == 8) && D < B, "blocks of 8 or 16 steps (within one selector word), deeper "
-                                             "than the prefetch");
+  static_assert((B == 16 || B == 8) && D <= B, "blocks of 8 or 16 steps (within one selector word), at least "
+                                              "as deep as the prefetch");
   // the prefetch registers form a ring indexed by the step within the block
   // modulo D, which carries over from block to block only if D divides B
   static_assert(B % D == 0, "prefetch depth must divide the block length");
