@@ -94,6 +94,8 @@ struct SceneDesc {
   float sfm1;            // (float)S - 1.0f    (fluid.rs:403)
   int has_src;           // frame sources to add: bit 0 density (and scratch), 1 vx, 2 vy
   int prio;              // wave issue priority 0..3 in a batch launch (largest scenes highest)
+  int frame_mult;        // frames per launch frame (> 1: a small scene of a server lane takes
+                         // several frames in one tick of the lane's largest); 0 or 1: one
 };
 
 // Exact-mode wavefront schedule: prefetch depth (steps) and steps between

@@ -10,7 +10,10 @@
 // iteration counts).  When a lane's tick ends, the thread retires the lane's
 // finished scenes, adds the ones admitted to it meanwhile and launches its
 // next tick, so a small scene's frame takes its own lane's time rather than
-// the largest resident scene's.  Between those events the thread admits
+// the largest resident scene's.  Within a wavefront lane's tick, a scene
+// whose frame chain is k times shorter than the lane's longest runs up to k
+// frames (at most 16, never past its request) in its workgroup, so the
+// lane's scenes advance at their own pace.  Between those events the thread admits
 // queued requests while their device memory fits the budget (allocation,
 // Fluid::new's state, obstacles, sources) on a fourth stream, overlapping the
 // running ticks.  Every scene runs its own workgroups with its own data, so a
@@ -50,6 +53,7 @@ struct Req {
   bool cancel = false;
   bool forget = false;  // released while queued or resident: dropped from the ticket map once retired
   int64_t frames_done = 0;
+  int tick_frames = 1;  // frames of the tick in flight
   double t_submit = 0, t_admit = 0, t_done = 0;
   eqf_scene* scene = nullptr;
   bool rb = false;  // red-black mode: advanced by its launch group on the red-black lane
@@ -78,7 +82,7 @@ struct Lane {
   // wavefront lanes: the descriptors of the scenes last launched
   SceneDesc* d_descs = nullptr;
   int desc_cap = 0;
-  std::vector<uint64_t> last_uids;  // scene uids (never reused) whose descriptors d_descs holds
+  std::vector<uint64_t> last_uids;  // (scene uid (never reused), frames per tick) pairs that d_descs holds
   std::vector<SceneDesc> descs;
 };
 
@@ -200,17 +204,34 @@ static void lane_tick_done(eqf_server* sv, Lane& L, double now) {
   const float ms = (float)(now - L.t_launch);
   sv->ticks++;
   for (Req* r : L.ticking) {
-    r->frames_done++;
-    sv->scene_frames++;
-    sv->cell_updates += (double)r->cells;
-    sv->frame_lat.emplace_back(r->p.size, ms);
+    const int f = r->tick_frames;
+    r->frames_done += f;
+    sv->scene_frames += f;
+    sv->cell_updates += (double)r->cells * f;
+    for (int k = 0; k < f; ++k) sv->frame_lat.emplace_back(r->p.size, ms / f);
   }
   L.ticking.clear();
   L.busy = false;
   if (--sv->busy_lanes == 0) sv->busy_ms += now - sv->busy_since;
 }
 
-// Launches one frame of every scene of an idle lane.  Returns an error
+// Dependent wavefront steps of one frame of a scene: its lin_solves' chains
+// (sweeps over m streams per strip, strip and stream fill); the one-pass
+// phases add a few passes over the strip.
+static double frame_chain(const eqf_scene* s) {
+  const int W = eqf_scene_waves(s->ns, s->P);
+  const int lag = cyc_stream_lag(EQF_GS_B, EQF_GS_D);
+  const int m = std::max(1, std::min(W / std::max(s->ns, 1), s->P / lag));
+  auto solve = [&](int it) {
+    if (it <= 0) return 0.0;
+    const int mm = std::min(m, it);
+    return (double)((it + mm - 1) / mm) * s->P + (double)(s->ns - 1) * cyc_strip_lag(EQF_GS_B, EQF_GS_D) +
+           (double)(mm - 1) * lag + 64.0;
+  };
+  return 3.0 * solve(s->desc.diff_iters) + 2.0 * solve(s->desc.press_iters) + 4.0 * s->P;
+}
+
+// Launches one tick of every scene of an idle lane.  Returns an error
 // message, or "" on success.
 static std::string lane_launch(eqf_server* sv, int li) {
   Lane& L = sv->lanes[li];
@@ -221,18 +242,27 @@ static std::string lane_launch(eqf_server* sv, int li) {
     auto& wf = L.ticking;
     std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
     int maxns = 1, lds = 0, waves = 1;
+    double chain_max = 0;
     for (Req* r : wf) {
       maxns = std::max(maxns, r->scene->ns);
       lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
       waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
+      chain_max = std::max(chain_max, frame_chain(r->scene));
     }
-    std::vector<uint64_t> uids(wf.size());
-    for (size_t i = 0; i < wf.size(); ++i) uids[i] = wf[i]->scene->uid;
+    std::vector<uint64_t> uids(2 * wf.size());
+    for (size_t i = 0; i < wf.size(); ++i) {
+      Req* r = wf[i];
+      const int64_t k = (int64_t)(chain_max / frame_chain(r->scene));
+      r->tick_frames = (int)std::max<int64_t>(1, std::min<int64_t>({k, 16, r->nframes - r->frames_done}));
+      uids[2 * i] = r->scene->uid;
+      uids[2 * i + 1] = (uint64_t)r->tick_frames;
+    }
     if (uids != L.last_uids) {
       L.descs.resize(wf.size());
       for (size_t i = 0; i < wf.size(); ++i) {
         L.descs[i] = wf[i]->scene->desc;
         L.descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
+        L.descs[i].frame_mult = wf[i]->tick_frames;
       }
       L.last_uids.clear();
       if ((int)wf.size() > L.desc_cap) {
@@ -255,7 +285,10 @@ static std::string lane_launch(eqf_server* sv, int li) {
     std::map<std::pair<int, int>, std::vector<eqf_scene*>> by_iters;
     std::stable_sort(L.ticking.begin(), L.ticking.end(),
                      [](const Req* a, const Req* b) { return a->cells > b->cells; });
-    for (Req* r : L.ticking) by_iters[{r->scene->desc.diff_iters, r->scene->desc.press_iters}].push_back(r->scene);
+    for (Req* r : L.ticking) {
+      r->tick_frames = 1;
+      by_iters[{r->scene->desc.diff_iters, r->scene->desc.press_iters}].push_back(r->scene);
+    }
     for (auto& [key, members] : by_iters) {
       void*& g = sv->rb_groups[key];
       if (!g) g = rb_batch_create(members, L.st, err);

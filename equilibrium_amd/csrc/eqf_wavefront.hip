@@ -730,7 +730,8 @@ __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* 
     case 1: __builtin_amdgcn_s_setprio(1); break;
     default: break;
   }
-  for (int fr = 0; fr < nframes; ++fr) eqf::frame_cyclic<D, B>(sc, (eqf::lf*)eqf_lds);
+  const int nf = sc.frame_mult > 1 ? nframes * sc.frame_mult : nframes;
+  for (int fr = 0; fr < nf; ++fr) eqf::frame_cyclic<D, B>(sc, (eqf::lf*)eqf_lds);
 }
 
 template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>(const eqf::SceneDesc* __restrict__, int);
