@@ -136,6 +136,34 @@ def test_server_red_black_requests_beside_wavefront():
     srv.close()
 
 
+def test_server_large_lane_runs_smaller_scenes_several_frames_per_tick():
+    """Wavefront scenes of 5-16 strips share a lane whose tick lasts one frame
+    of its longest frame chain (N=1000 here); the N=300 and N=520 scenes run
+    several frames per tick, never past their request (frame counts that are
+    not multiples of the frames per tick).  Every result equals its solo run
+    bit for bit, and their frame latency is below the N=1000 scene's."""
+    _case("server large lane, several frames per tick")
+    specs = [(1000, 3), (300, 7), (520, 13), (300, 2)]
+    mix = [scenes.stam_scene(N, scenes.derive_seed(35, i), iters=6, frames=fr, n_rects=1)
+           for i, (N, fr) in enumerate(specs)]
+    srv = _abi.Server(hbm_budget_bytes=16 << 30)
+    tickets = [srv.submit(sc) for sc in mix]
+    srv.wait_idle(120000)
+    st = srv.stats()
+    assert st["scene_frames"] == sum(fr for _, fr in specs)
+    for sc, t in zip(mix, tickets):
+        s = srv.poll(t)
+        assert s.state == _abi.REQ_DONE and s.frames_done == sc["frames"]
+        d, _ = _solo(sc)
+        assert_bit_equal(srv.result(t, _abi.DENSITY), d, f"density N={sc['size'] - 2}")
+    small = srv.stats(min_size=302, max_size=522)
+    big = srv.stats(min_size=1002, max_size=1002)
+    print(f"[serving] p50 frame N=300/520 {small['p50_frame_ms']:.2f} ms, N=1000 {big['p50_frame_ms']:.2f} ms",
+          flush=True)
+    assert 0 < small["p50_frame_ms"] < big["p50_frame_ms"]
+    srv.close()
+
+
 def test_server_churn_results_equal_solo_runs():
     """Rounds of submissions while earlier requests are resident: finished
     tickets are released (their records freed) and new requests take their
