@@ -128,3 +128,31 @@ def test_batch_rejects_a_scene_listed_twice():
     h = ctypes.c_void_p()
     assert L.eqf_batch_create(ctypes.addressof(fake_ctx), arr, 3, ctypes.byref(h)) == _abi.EQF_EINVAL
     assert "more than once" in _abi.last_error()
+
+
+def test_scene_parameter_limits_rejected_before_the_device():
+    """eqf_scene_create validates the parameters before any device call (the
+    context below is never dereferenced): size >= 3 (one interior cell inside
+    the wall ring, fluid.rs:552-570), wavefront mode up to size 1026 (16 strips
+    of 64 rows in one workgroup), a known GS mode, init run once or twice
+    (Fluid::new / Fluid::default, fluid.rs:83-110), bounded iteration counts."""
+    L = _abi.lib()
+    fake_ctx = ctypes.create_string_buffer(64)
+    h = ctypes.c_void_p()
+
+    def create(**kw):
+        p = _abi.default_params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return L.eqf_scene_create(ctypes.addressof(fake_ctx), ctypes.byref(p), ctypes.byref(h))
+
+    cases = [({"size": 2}, "size must be >= 3"),
+             ({"size": 1027, "gs_mode": _abi.GS_WAVEFRONT}, "size <= 1026"),
+             ({"size": 40000}, "size must be <= 32768"),
+             ({"gs_mode": 7}, "gs_mode"),
+             ({"init_times": 3}, "init_times"),
+             ({"gs_iters": 1 << 25}, "iteration count too large")]
+    for kw, msg in cases:
+        assert create(**kw) == _abi.EQF_EINVAL, kw
+        assert msg in _abi.last_error(), (kw, _abi.last_error())
+    assert not h.value
