@@ -80,3 +80,12 @@ def test_scene_calls_between_batch_steps(mode):
     b.close()
     for h in hs:
         h.close()
+
+
+def test_device_key_names_the_gpu():
+    """eqf_ctx_device_key: "<name>/<gfx arch>/<CU count>CU", the tuning caches' key."""
+    _case("device key")
+    key = _abi.default_context().device_key()
+    print(f"[device] {key}", flush=True)
+    name, arch, cus = key.rsplit("/", 2)
+    assert name and arch.startswith("gfx950") and cus.endswith("CU") and int(cus[:-2]) > 0
