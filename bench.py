@@ -295,14 +295,13 @@ def bench_single_dropin(scs, frames):
     return wall
 
 
-def cpu_single_1024(frames=3):
-    """The oracle's exact (lexicographic f32) step on the N=1024 grid on one
-    core, as the reference runs it: one warm-up frame, then `frames` frames
-    timed one by one."""
+def cpu_single(sc, frames=3):
+    """The oracle's exact (lexicographic f32) step on one grid on one core, as
+    the reference runs it (one thread, renderer.rs:126-128): one warm-up
+    frame, then `frames` frames timed one by one."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from helpers import oracle_state
 
-    sc = scenes.single_1024()
     st = oracle_state(sc)
 
     def frame():
@@ -318,7 +317,7 @@ def cpu_single_1024(frames=3):
     N = sc["size"] - 2
     med = sorted(times)[len(times) // 2]
     return {"value": N * N / med, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "median_frame_s": med, "total_s": sum(times),
+            "median_frame_s": med, "total_s": sum(times), "gs_sweeps": sc["iters"],
             "sample": f"N={N} exact step, 1 warm-up + {frames} timed frames on one core"}
 
 
@@ -677,6 +676,8 @@ def main():
             "roofline": {
                 "bound": "hbm",
                 "kernel": "eqf_frames_cyclic (fused frame step, one workgroup per scene)",
+                "frac_basis": "algorithmic bytes (B_alg per cell-update) / launch time; traffic_frac: "
+                              "measured HBM bytes (rocprofv3) / launch time",
                 "achieved": achieved,
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
@@ -688,6 +689,7 @@ def main():
         shape = f"{tuning['max_waves']}:{0 if tuning['order'] == 'largest first' else 1}"
         traffic, ent = profiled_traffic("eqf_frames_cyclic", shape)
         out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_frac"] = (traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None
         out["roofline"]["traffic_source"] = (
             f"{ent['source']} at launch shape {shape} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE, scaled by a copy "
             "of known size)" if ent else f"no rocprofv3 profile at launch shape {shape}")
@@ -716,6 +718,19 @@ def main():
         log(f"single drop-in: {single:.4g} cell-updates/s")
 
     if "n1024" in legs:
+        # N=128 single grid (the reference's own CPU-runnable case), exact mode, all 100 frames
+        c0 = scenes.small_scene()
+        h = make_scenes([c0])[0]
+        h.step(1)
+        h.sync()
+        t0 = time.perf_counter()
+        h.step(c0["frames"])
+        h.sync()
+        w0 = time.perf_counter() - t0
+        out["n128_single_wavefront"] = {"value": 128 * 128 * c0["frames"] / w0, "unit": "cell-updates/s",
+                                        "ms_per_frame": w0 * 1e3 / c0["frames"], "frames_timed": c0["frames"]}
+        h.close()
+        log(f"N=128 single: {out['n128_single_wavefront']}")
         # N=1024 single grid, exact mode
         c1 = scenes.single_1024()
         h = make_scenes([c1])[0]
@@ -772,8 +787,20 @@ def main():
                                "sample": f"all {len(mix)} scenes of the mix x {cframes} frame(s) after one untimed "
                                          f"warm-up frame, one scene per thread, {wcpu:.1f} s"}
         log(f"cpu oracle: {rate:.4g} cell-updates/s on {threads} threads ({wcpu:.1f} s)")
-        out["cpu_single_n1024"] = cpu_single_1024()
+        out["cpu_single_n1024"] = cpu_single(scenes.single_1024())
         log(f"cpu oracle N=1024 on one core: {out['cpu_single_n1024']}")
+        out["cpu_single_n128"] = cpu_single(scenes.small_scene(), frames=10)
+        log(f"cpu oracle N=128 on one core: {out['cpu_single_n128']}")
+
+    if "cpu4096" in legs:
+        # The N=4096 grid on one core, as the reference would run it: ~30 s per
+        # frame, so opt-in (not in the default legs); GPU frame time beside it
+        # from the rb4096 leg when that ran
+        c = cpu_single(scenes.redblack_scene(), frames=3)
+        if "n4096_single_redblack" in out:
+            c["gpu_over_cpu"] = out["n4096_single_redblack"]["value"] / c["value"]
+        out["cpu_single_n4096"] = c
+        log(f"cpu oracle N=4096 on one core: {c}")
 
     print(json.dumps(out), flush=True)
 

@@ -200,7 +200,13 @@ extern "C" int eqf_ctx_device_key(const eqf_ctx* ctx, char* buf, size_t len) {
   if (!ctx || !buf || len == 0) return fail(EQF_EINVAL, "ctx or buf is NULL, or len is 0");
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
-  const int n = snprintf(buf, len, "%s/%s/%dCU", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  // the properties' name can be empty (seen on MI355X): then the driver's
+  // name, else a placeholder; the architecture and CU count follow anyway
+  char name[256] = {0};
+  std::snprintf(name, sizeof(name), "%s", prop.name);
+  if (!name[0] && hipDeviceGetName(name, (int)sizeof(name), ctx->device) != hipSuccess) name[0] = 0;
+  if (!name[0]) std::snprintf(name, sizeof(name), "gpu");
+  const int n = snprintf(buf, len, "%s/%s/%dCU", name, prop.gcnArchName, prop.multiProcessorCount);
   if (n < 0 || (size_t)n >= len) {
     buf[0] = 0;
     return fail(EQF_EINVAL, "buffer too small for the device key");

@@ -83,8 +83,9 @@ const char *eqf_last_error(void);
 int eqf_ctx_create(int device, eqf_ctx **out);
 int eqf_ctx_destroy(eqf_ctx *ctx);
 /* No counterpart in the reference (host plumbing for tuning caches): writes
- * "<device name>/<gfx arch>/<CU count>" of the context's GPU, NUL-terminated,
- * into buf; EQF_EINVAL if it does not fit in len bytes. */
+ * "<device name>/<gfx arch>/<CU count>CU" of the context's GPU, NUL-terminated,
+ * into buf ("gpu" when the driver reports no name); EQF_EINVAL if it does not
+ * fit in len bytes. */
 int eqf_ctx_device_key(const eqf_ctx *ctx, char *buf, size_t len);
 
 /* Fluid::new / Fluid::default (fluid.rs:83-110): allocates the six fields and
