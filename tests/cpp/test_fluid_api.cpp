@@ -1,6 +1,7 @@
 // The reference's own unit tests, restated against the C++ mirror of its API
 // (equilibrium_amd/host/fluid.hpp) over libeqfluid.so, plus a step-parity
-// check against the CPU oracle.  `--cpu` runs the tests that need no GPU.
+// check against the CPU oracle, and the batch and serving entry points
+// against solo runs.  `--cpu` runs the tests that need no GPU.
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -121,6 +122,110 @@ static void snapshots_match_oracle() {
   }
 }
 
+// The batch and serving entry points from C++ through the raw C ABI: three
+// scenes of different sizes (1, 2 and 5 strips) with per-frame density and
+// velocity sources and an obstacle, advanced (a) alone, (b) in one batch,
+// (c) as server requests; all three give the same fields bit for bit.
+static std::vector<float> ramp(uint32_t S, float scale) {
+  std::vector<float> v((size_t)S * S);
+  for (size_t i = 0; i < v.size(); ++i) v[i] = scale * (float)((i * 7919u) % 97u) / 97.0f;
+  return v;
+}
+
+static eqf_scene* make_scene(uint32_t S, const std::vector<float>& sd, const std::vector<float>& su) {
+  eqf_params p;
+  eqf_params_default(&p);
+  p.size = S;
+  p.delta_t = 0.1f;
+  p.viscosity = 1e-4f;
+  p.diffusion = 1e-5f;
+  p.gs_iters = 6;
+  eqf_scene* h = nullptr;
+  detail::check(eqf_scene_create(detail::default_ctx(), &p, &h));
+  detail::check(eqf_fill_rect(h, S / 4, S / 4, S / 4 + 5, S / 4 + 7));
+  detail::check(eqf_set_frame_source(h, EQF_DENSITY, sd.data()));
+  detail::check(eqf_set_frame_source(h, EQF_VX, su.data()));
+  return h;
+}
+
+static void batch_and_server_match_solo() {
+  const uint32_t sizes[3] = {50, 100, 300};
+  const int64_t frames = 3;
+  std::vector<std::vector<float>> solo_d, batch_d, server_d;
+  std::vector<std::vector<float>> sd, su;
+  for (uint32_t S : sizes) {
+    sd.push_back(ramp(S, 2.0f));
+    su.push_back(ramp(S, 0.5f));
+  }
+  for (int i = 0; i < 3; ++i) {  // (a) alone
+    eqf_scene* h = make_scene(sizes[i], sd[i], su[i]);
+    detail::check(eqf_step(h, frames));
+    std::vector<float> d((size_t)sizes[i] * sizes[i]);
+    detail::check(eqf_download(h, EQF_DENSITY, d.data()));
+    solo_d.push_back(d);
+    detail::check(eqf_scene_destroy(h));
+  }
+  {  // (b) one batch
+    eqf_scene* hs[3];
+    for (int i = 0; i < 3; ++i) hs[i] = make_scene(sizes[i], sd[i], su[i]);
+    eqf_batch* b = nullptr;
+    detail::check(eqf_batch_create(detail::default_ctx(), hs, 3, &b));
+    detail::check(eqf_batch_step(b, frames));
+    detail::check(eqf_batch_sync(b));
+    double ms = 0;
+    detail::check(eqf_batch_last_ms(b, &ms));
+    CHECK(ms > 0);
+    detail::check(eqf_batch_destroy(b));
+    for (int i = 0; i < 3; ++i) {
+      std::vector<float> d((size_t)sizes[i] * sizes[i]);
+      detail::check(eqf_download(hs[i], EQF_DENSITY, d.data()));
+      batch_d.push_back(d);
+      detail::check(eqf_scene_destroy(hs[i]));
+    }
+  }
+  {  // (c) server requests
+    eqf_server_config cfg{};
+    cfg.hbm_budget_bytes = 4ull << 30;
+    eqf_server* sv = nullptr;
+    detail::check(eqf_server_create(detail::default_ctx(), &cfg, &sv));
+    uint64_t tickets[3];
+    for (int i = 0; i < 3; ++i) {
+      eqf_request rq{};
+      eqf_params_default(&rq.params);
+      rq.params.size = sizes[i];
+      rq.params.delta_t = 0.1f;
+      rq.params.viscosity = 1e-4f;
+      rq.params.diffusion = 1e-5f;
+      rq.params.gs_iters = 6;
+      rq.nframes = frames;
+      const int64_t rect[4] = {sizes[i] / 4, sizes[i] / 4, sizes[i] / 4 + 5, sizes[i] / 4 + 7};
+      rq.n_rects = 1;
+      rq.rects = rect;
+      rq.src_density = sd[i].data();
+      rq.src_vx = su[i].data();
+      detail::check(eqf_server_submit(sv, &rq, &tickets[i]));  // inputs copied at submit
+    }
+    detail::check(eqf_server_wait_idle(sv, 60000.0));
+    for (int i = 0; i < 3; ++i) {
+      eqf_request_status st{};
+      detail::check(eqf_server_poll(sv, tickets[i], &st));
+      CHECK(st.state == EQF_REQ_DONE && st.frames_done == frames);
+      std::vector<float> d((size_t)sizes[i] * sizes[i]);
+      detail::check(eqf_server_result(sv, tickets[i], EQF_DENSITY, d.data()));
+      server_d.push_back(d);
+      detail::check(eqf_server_release(sv, tickets[i]));
+    }
+    eqf_server_stats st{};
+    detail::check(eqf_server_stats_get(sv, &st));
+    CHECK(st.scene_frames == 3 * frames);
+    detail::check(eqf_server_destroy(sv));
+  }
+  for (int i = 0; i < 3; ++i) {
+    CHECK(std::memcmp(solo_d[i].data(), batch_d[i].data(), solo_d[i].size() * 4) == 0);
+    CHECK(std::memcmp(solo_d[i].data(), server_d[i].data(), solo_d[i].size() * 4) == 0);
+  }
+}
+
 int main(int argc, char** argv) {
   const bool gpu = !(argc > 1 && std::strcmp(argv[1], "--cpu") == 0);
   to_coordinate();
@@ -129,6 +234,7 @@ int main(int argc, char** argv) {
     mark_fluid_obstacles_default_current_simulation();
     default_simulation_matches_oracle();
     snapshots_match_oracle();
+    batch_and_server_match_solo();
   }
   std::printf("%s: %d failure(s)\n", gpu ? "cpu+gpu" : "cpu", failures);
   return failures ? 1 : 0;
