@@ -1,0 +1,148 @@
+"""Brute-force pins of the oracle's set_boundaries and lexicographic
+lin_solve: a naive per-cell replay of the reference's loops in pure Python
+(one np.float32 operation at a time; fluid.rs:133-190 and 301-325) on tiny
+grids with random interior walls, against the C oracle bit for bit.
+
+The replay follows the reference literally: set_boundaries visits every cell
+(j outer, i inner), skips walls, reads clamped neighbours, writes in place,
+and averages the corners after the scan; lin_solve updates every interior
+cell (walls included) in lexicographic order with the reference's sum order
+and 1/c, then applies set_boundaries after every sweep.  A wrong index, a
+swapped precedence (right wins in AdjustRow, up wins in AdjustColumn) or an
+update order that reads other neighbours' new values in the oracle's loops
+fails it.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+WALL, FLUID = 0, 1
+ROW, COL, PASSIVE = 0, 1, 2
+F32 = np.float32
+
+
+def _mask(S, rng, wall_frac):
+    m = np.full(S * S, FLUID, np.uint8)
+    for j in range(S):
+        for i in range(S):
+            if i in (0, S - 1) or j in (0, S - 1) or rng.random() < wall_frac:
+                m[i + j * S] = WALL
+    return m
+
+
+def replay_set_bnd(mode, x, m, S):
+    """fluid.rs:133-190 (manage_single_cell_boundary over every cell, then the corners)."""
+    def k(i, j):
+        return i + j * S
+
+    def cl(v):
+        return min(max(v, 0), S - 1)
+
+    for j in range(S):
+        for i in range(S):
+            if m[k(i, j)] == WALL:
+                continue
+            up, down = (i, cl(j - 1)), (i, cl(j + 1))
+            left, right = (cl(i - 1), j), (cl(i + 1), j)
+            if mode == ROW:
+                if m[k(*left)] == WALL:
+                    x[k(i, j)] = -x[k(*left)]
+                if m[k(*right)] == WALL:
+                    x[k(i, j)] = -x[k(*right)]
+            elif mode == COL:
+                if m[k(*down)] == WALL:
+                    x[k(i, j)] = -x[k(*down)]
+                if m[k(*up)] == WALL:
+                    x[k(i, j)] = -x[k(*up)]
+            else:
+                x[k(i, 0)] = x[k(i, 1)]
+                x[k(i, S - 1)] = x[k(i, S - 2)]
+                x[k(0, j)] = x[k(1, j)]
+                x[k(S - 1, j)] = x[k(S - 2, j)]
+    h = F32(0.5)
+    x[k(0, 0)] = h * (x[k(1, 0)] + x[k(0, 1)])
+    x[k(0, S - 1)] = h * (x[k(1, S - 1)] + x[k(0, S - 2)])
+    x[k(S - 1, 0)] = h * (x[k(S - 2, 0)] + x[k(S - 1, 1)])
+    x[k(S - 1, S - 1)] = h * (x[k(S - 2, S - 1)] + x[k(S - 1, S - 2)])
+
+
+def replay_lin_solve(mode, x, x0, a, c, m, S, iters):
+    """fluid.rs:301-325."""
+    a = F32(a)
+    cr = F32(1.0) / F32(c)
+    for _ in range(iters):
+        for j in range(1, S - 1):
+            for i in range(1, S - 1):
+                t = i + j * S
+                s = ((x[t + 1] + x[t - 1]) + x[t + S]) + x[t - S]
+                x[t] = (x0[t] + a * s) * cr
+        replay_set_bnd(mode, x, m, S)
+
+
+def _same(got, want):
+    return np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", [ROW, COL, PASSIVE])
+def test_set_bnd_equals_naive_scan_on_random_masks(mode):
+    rng = np.random.default_rng(70 + mode)
+    for case in range(40):
+        S = int(rng.integers(3, 10))
+        m = _mask(S, rng, wall_frac=(0.0, 0.15, 0.35, 0.6)[case % 4])
+        x = rng.standard_normal(S * S).astype(F32)
+        want = x.copy()
+        replay_set_bnd(mode, want, m, S)
+        got = x.copy()
+        oracle.set_bnd(mode, got, S, m)
+        assert _same(got, want), (mode, case, S)
+
+
+@pytest.mark.parametrize("mode", [ROW, COL, PASSIVE])
+def test_lexicographic_lin_solve_equals_naive_replay(mode):
+    rng = np.random.default_rng(80 + mode)
+    for case in range(24):
+        S = int(rng.integers(3, 9))
+        m = _mask(S, rng, wall_frac=(0.0, 0.2, 0.4)[case % 3])
+        x = rng.standard_normal(S * S).astype(F32)
+        x0 = rng.standard_normal(S * S).astype(F32)
+        a = F32(rng.uniform(0.05, 3.0))
+        c = F32(1.0) + F32(4.0) * a
+        iters = int(rng.integers(1, 5))
+        want = x.copy()
+        replay_lin_solve(mode, want, x0, a, c, m, S, iters)
+        got = x.copy()
+        oracle.lin_solve(mode, got, x0, a, c, S, iters, m, order=oracle.LEX)
+        assert _same(got, want), (mode, case, S, iters)
+
+
+def test_replay_rejects_a_planted_order_change():
+    """The replay distinguishes the reference's lexicographic order from a
+    reversed scan (new right and lower neighbours instead of new left and
+    upper ones), so an order slip in the oracle would show.  (A column-major
+    scan would not: with a 5-point stencil it reads the same new and old
+    neighbours as the row-major one, which is what lets the anti-diagonal
+    wavefront reproduce the reference bit for bit.)"""
+    rng = np.random.default_rng(90)
+    S = 7
+    m = _mask(S, rng, wall_frac=0.0)
+    x = rng.standard_normal(S * S).astype(F32)
+    x0 = rng.standard_normal(S * S).astype(F32)
+    a, c = F32(1.0), F32(5.0)
+    lex = x.copy()
+    replay_lin_solve(PASSIVE, lex, x0, a, c, m, S, 1)
+    rev = x.copy()
+    cr = F32(1.0) / c
+    for j in range(S - 2, 0, -1):
+        for i in range(S - 2, 0, -1):
+            t = i + j * S
+            rev[t] = (x0[t] + a * (((rev[t + 1] + rev[t - 1]) + rev[t + S]) + rev[t - S])) * cr
+    replay_set_bnd(PASSIVE, rev, m, S)
+    assert not _same(lex, rev)
+    colmajor = x.copy()
+    for i in range(1, S - 1):
+        for j in range(1, S - 1):
+            t = i + j * S
+            colmajor[t] = (x0[t] + a * (((colmajor[t + 1] + colmajor[t - 1]) + colmajor[t + S]) + colmajor[t - S])) * cr
+    replay_set_bnd(PASSIVE, colmajor, m, S)
+    assert _same(lex, colmajor)
