@@ -1,7 +1,8 @@
-"""Brute-force pins of the oracle's set_boundaries and lexicographic
-lin_solve: a naive per-cell replay of the reference's loops in pure Python
-(one np.float32 operation at a time; fluid.rs:133-190 and 301-325) on tiny
-grids with random interior walls, against the C oracle bit for bit.
+"""Brute-force pins of the oracle's set_boundaries, lexicographic
+lin_solve, project and advect: a naive per-cell replay of the reference's
+loops in pure Python (one np.float32 operation at a time; fluid.rs:133-190,
+301-325, 329-372 and 375-432) on tiny grids with random interior walls,
+against the C oracle bit for bit.
 
 The replay follows the reference literally: set_boundaries visits every cell
 (j outer, i inner), skips walls, reads clamped neighbours, writes in place,
@@ -146,3 +147,99 @@ def test_replay_rejects_a_planted_order_change():
             colmajor[t] = (x0[t] + a * (((colmajor[t + 1] + colmajor[t - 1]) + colmajor[t + S]) + colmajor[t - S])) * cr
     replay_set_bnd(PASSIVE, colmajor, m, S)
     assert _same(lex, colmajor)
+
+
+def replay_project(u, v, p, div, m, S, iters):
+    """fluid.rs:329-372."""
+    sf = F32(S)
+    for j in range(1, S - 1):
+        for i in range(1, S - 1):
+            t = i + j * S
+            div[t] = (F32(-0.5) * (((u[t + 1] - u[t - 1]) + v[t + S]) - v[t - S])) / sf
+            p[t] = F32(0.0)
+    replay_set_bnd(PASSIVE, div, m, S)
+    replay_set_bnd(PASSIVE, p, m, S)
+    replay_lin_solve(PASSIVE, p, div, F32(1.0), F32(4.0), m, S, iters)
+    for j in range(1, S - 1):
+        for i in range(1, S - 1):
+            t = i + j * S
+            u[t] = u[t] - (F32(0.5) * (p[t + 1] - p[t - 1])) * sf
+            v[t] = v[t] - (F32(0.5) * (p[t + S] - p[t - S])) * sf
+    replay_set_bnd(ROW, u, m, S)
+    replay_set_bnd(COL, v, m, S)
+
+
+def replay_advect(mode, d, d0, u, v, dt, m, S):
+    """fluid.rs:375-432, the row `break` included."""
+    dtx = F32(dt) * F32(S - 2)
+    sf = F32(S)
+    for j in range(1, S - 1):
+        for i in range(1, S - 1):
+            t = i + j * S
+            x = F32(i) - dtx * u[t]
+            y = F32(j) - dtx * v[t]
+            x = min(max(x, F32(0.5)), sf - F32(1.0))  # f32::clamp (no NaN in these inputs)
+            y = min(max(y, F32(0.5)), sf - F32(1.0))
+            i0 = np.floor(x)
+            i1 = i0 + F32(1.0)
+            j0 = np.floor(y)
+            j1 = j0 + F32(1.0)
+            s1 = x - i0
+            s0 = F32(1.0) - s1
+            t1 = y - j0
+            t0 = F32(1.0) - t1
+            if i1 >= sf or j1 >= sf:
+                d[t] = d[t - 1]
+                break
+            a0, a1, b0, b1 = int(i0), int(i1), int(j0), int(j1)
+            d[t] = s0 * (t0 * d0[a0 + b0 * S] + t1 * d0[a0 + b1 * S]) + \
+                s1 * (t0 * d0[a1 + b0 * S] + t1 * d0[a1 + b1 * S])
+    replay_set_bnd(mode, d, m, S)
+
+
+def test_project_equals_naive_replay():
+    rng = np.random.default_rng(100)
+    for case in range(24):
+        S = int(rng.integers(4, 9))
+        m = _mask(S, rng, wall_frac=(0.0, 0.2)[case % 2])
+        u = rng.standard_normal(S * S).astype(F32)
+        v = rng.standard_normal(S * S).astype(F32)
+        p = rng.standard_normal(S * S).astype(F32)
+        div = rng.standard_normal(S * S).astype(F32)
+        iters = int(rng.integers(1, 5))
+        want = [a.copy() for a in (u, v, p, div)]
+        replay_project(*want, m, S, iters)
+        got = [a.copy() for a in (u, v, p, div)]
+        oracle.project(*got, S, iters, m, order=oracle.LEX)
+        for g, w, name in zip(got, want, ("u", "v", "p", "div")):
+            assert _same(g, w), (case, S, name)
+
+
+@pytest.mark.parametrize("mode", [ROW, COL, PASSIVE])
+def test_advect_equals_naive_replay_with_breaks(mode):
+    rng = np.random.default_rng(110 + mode)
+    breaks = 0
+    for case in range(30):
+        S = int(rng.integers(4, 10))
+        m = _mask(S, rng, wall_frac=(0.0, 0.25)[case % 2])
+        scale = (0.5, 3.0, 12.0)[case % 3]  # large velocities back-trace past the last column
+        u = (scale * rng.standard_normal(S * S)).astype(F32)
+        v = (scale * rng.standard_normal(S * S)).astype(F32)
+        d = rng.standard_normal(S * S).astype(F32)
+        d0 = rng.standard_normal(S * S).astype(F32)
+        dt = F32(rng.uniform(0.02, 0.3))
+        want = d.copy()
+        replay_advect(mode, want, d0, u, v, dt, m, S)
+        got = d.copy()
+        oracle.advect(mode, got, d0, u, v, S, dt, m)
+        assert _same(got, want), (mode, case, S)
+        sf = F32(S)
+        for j in range(1, S - 1):  # count rows whose back-trace hits the clamp's upper end
+            for i in range(1, S - 1):
+                t = i + j * S
+                x = min(max(F32(i) - F32(dt) * F32(S - 2) * u[t], F32(0.5)), sf - F32(1.0))
+                y = min(max(F32(j) - F32(dt) * F32(S - 2) * v[t], F32(0.5)), sf - F32(1.0))
+                if np.floor(x) + F32(1.0) >= sf or np.floor(y) + F32(1.0) >= sf:
+                    breaks += 1
+                    break
+    assert breaks > 10  # the row break is exercised
