@@ -1,8 +1,8 @@
 """Brute-force pins of the oracle's set_boundaries, lexicographic
-lin_solve, project and advect: a naive per-cell replay of the reference's
-loops in pure Python (one np.float32 operation at a time; fluid.rs:133-190,
-301-325, 329-372 and 375-432) on tiny grids with random interior walls,
-against the C oracle bit for bit.
+lin_solve, project, advect and whole frames: a naive per-cell replay of the
+reference's loops in pure Python (one np.float32 operation at a time;
+fluid.rs:133-190, 276-325, 329-372, 375-432 and 437-524) on tiny grids with
+random interior walls, against the C oracle bit for bit.
 
 The replay follows the reference literally: set_boundaries visits every cell
 (j outer, i inner), skips walls, reads clamped neighbours, writes in place,
@@ -243,3 +243,54 @@ def test_advect_equals_naive_replay_with_breaks(mode):
                     breaks += 1
                     break
     assert breaks > 10  # the row break is exercised
+
+
+def replay_diffuse(mode, x, x0, k, dt, m, S, iters):
+    """fluid.rs:276-298: a = dt * k * (S-2) * (S-2), c = 1 + 4a."""
+    sf = F32(S - 2)
+    a = ((F32(dt) * F32(k)) * sf) * sf
+    replay_lin_solve(mode, x, x0, a, F32(1.0) + F32(4.0) * a, m, S, iters)
+
+
+def replay_frame(f, m, S, dt, diff, visc, iters, sd, su, sv):
+    """The frame loop's add-then-step (renderer_helpers.rs:55-59: sources
+    added as add_density / add_velocity do, fluid.rs:120-131), then step
+    (fluid.rs:437-524) with its buffer roles."""
+    D, Sc, U, V, U0, V0 = (f[k] for k in oracle.FIELDS)
+    D += sd
+    Sc += sd
+    U += su
+    V += sv
+    replay_diffuse(ROW, U0, U, visc, dt, m, S, iters)
+    replay_diffuse(COL, V0, V, visc, dt, m, S, iters)
+    replay_project(U0, V0, U, V, m, S, iters)
+    replay_advect(ROW, U, U0, U0, V0, dt, m, S)
+    replay_advect(COL, V, V0, U0, V0, dt, m, S)
+    replay_project(U, V, U0, V0, m, S, iters)
+    replay_diffuse(PASSIVE, Sc, D, diff, dt, m, S, iters)
+    replay_advect(PASSIVE, D, Sc, U, V, dt, m, S)
+    Sc[:] = D
+
+
+def test_whole_frames_equal_naive_replay():
+    """Three frames with sources, obstacles and random fields: the oracle's
+    step (its buffer roles included) equals the replay composed from the
+    component replays above, all six fields bit for bit."""
+    rng = np.random.default_rng(120)
+    for case in range(6):
+        S = int(rng.integers(6, 11))
+        st = oracle.State.new(S)
+        st.mask[:] = _mask(S, rng, wall_frac=(0.0, 0.15)[case % 2])
+        for k in oracle.FIELDS:
+            st.f[k][:] = rng.standard_normal(S * S).astype(F32)
+        sd = rng.standard_normal(S * S).astype(F32)
+        su = (2.0 * rng.standard_normal(S * S)).astype(F32)
+        sv = (2.0 * rng.standard_normal(S * S)).astype(F32)
+        dt, diff, visc = F32(rng.uniform(0.05, 0.2)), F32(rng.uniform(0, 1e-2)), F32(rng.uniform(0, 1e-2))
+        iters = int(rng.integers(1, 5))
+        want = {k: st.f[k].copy() for k in oracle.FIELDS}
+        for _ in range(3):
+            replay_frame(want, st.mask, S, dt, diff, visc, iters, sd, su, sv)
+        st.run(dt, diff, visc, iters, 3, src_d=sd, src_u=su, src_v=sv)
+        for k in oracle.FIELDS:
+            assert _same(st.f[k], want[k]), (case, S, k)
