@@ -294,3 +294,46 @@ def test_whole_frames_equal_naive_replay():
         st.run(dt, diff, visc, iters, 3, src_d=sd, src_u=su, src_v=sv)
         for k in oracle.FIELDS:
             assert _same(st.f[k], want[k]), (case, S, k)
+
+
+def replay_init(S, times):
+    """Fluid::new (times=1) / Fluid::default (times=2): init_velocities,
+    init_density, init_walls (fluid.rs:84-117, 527-571, 602-606) through
+    add_velocity / add_density (fluid.rs:120-131), with the reference's u32
+    range bounds (S/2 - 10 wraps below 20: an empty range) and idx!'s
+    clamped coordinates (fluid.rs:31-35)."""
+    f = {k: np.zeros(S * S, F32) for k in oracle.FIELDS}
+    mask = np.full(S * S, FLUID, np.uint8)
+
+    def idx(x, y):
+        return min(max(x, 0), S - 1) + min(max(y, 0), S - 1) * S
+
+    for _ in range(times):
+        for j in range(S):
+            for i in range(S):
+                f["vx"][idx(i, j)] += F32(1.0)
+                f["vy"][idx(i, j)] += F32(1.0)
+        for j in range(S):
+            for i in range(S):
+                f["density"][idx(i, j)] += F32(0.0)
+                f["scratch"][idx(i, j)] += F32(0.0)
+        lo, hi = (S // 2 - 10) % (1 << 32), S // 2 + 10
+        for j in range(lo, hi + 1):
+            for i in range(lo, hi + 1):
+                f["density"][idx(i, j)] += F32(0.9)
+                f["scratch"][idx(i, j)] += F32(0.9)
+        for i in range(S):
+            mask[idx(i, 0)] = mask[idx(i, S - 1)] = WALL
+        for j in range(S):
+            mask[idx(0, j)] = mask[idx(S - 1, j)] = WALL
+    return f, mask
+
+
+@pytest.mark.parametrize("times", [1, 2])
+def test_init_equals_naive_replay_for_many_sizes(times):
+    for S in list(range(3, 44)) + [64, 128]:
+        f, mask = replay_init(S, times)
+        st = oracle.State.new(S, times)
+        assert np.array_equal(st.mask, mask), S
+        for k in oracle.FIELDS:
+            assert _same(st.f[k], f[k]), (S, k)
