@@ -51,7 +51,7 @@ enum CodeBits : uint32_t {
 };
 
 // Per-cell rule byte of a ROWMAJOR scene (one byte per cell, ring included,
-// laid out like a field), read by the red-black sweeps:
+// cell (x, y) at y * rm_code_stride(S) + x), read by the red-black sweeps:
 //   bit 0     interior cell (updated by the sweeps)
 //   bits 1-2  AdjustRow: 0 keep, 1 -(left), 2 -(right)  (right wins, fluid.rs:151-164)
 //   bits 3-4  AdjustColumn: 0 keep, 1 -(up), 2 -(down)  (up wins, fluid.rs:165-177)
@@ -79,7 +79,7 @@ enum Sel : uint32_t { SEL_KEEP = 0, SEL_NEG1 = 1, SEL_NEG2 = 2, SEL_FIXUP = 3 };
 struct SceneDesc {
   float* fld;            // NFIELDS fields, `fstride` floats apart
   const float* src;      // per-frame sources (density, vx, vy) or nullptr
-  const uint8_t* code;   // cyclic: 3 selector arrays (Sel); row-major: CodeBits at x + y*S
+  const uint8_t* code;   // cyclic: 3 selector arrays (Sel); row-major: RuleBits at x + y*rm_code_stride(S)
   const uint8_t* rowf;   // [S] row j holds a NoWall cell
   const int2* bnd;       // (dst, src) field offsets: ROW list, COL list, PASSIVE list
   int64_t fstride;       // floats per field
@@ -185,5 +185,10 @@ EQF_HD inline int64_t rm_field_floats(int S) {
   int64_t n = (int64_t)S * S;
   return (n + 63) & ~(int64_t)63;
 }
+
+// Row-major rule bytes (RuleBits): rows padded to an even length, so that a
+// lane's column pair (even first column) is one 2-byte-aligned load for any S.
+EQF_HD inline int rm_code_stride(int S) { return S + (S & 1); }
+EQF_HD inline int64_t rm_code_bytes(int S) { return (int64_t)S * rm_code_stride(S); }
 
 }  // namespace eqf

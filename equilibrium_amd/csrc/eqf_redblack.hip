@@ -203,33 +203,42 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
   const int gyb = by * TY;
   const int gy0 = gyb + w * R;          // this wave's band
   const int gxa = gx0 + 2 * lane;
-  // this lane's two columns, clamped into the grid (cells outside it are
-  // replaced by 0 after the loads), and whether they are in the grid
-  uint32_t col[2];
+  // whether this lane's two columns are in the grid (cells outside it are
+  // replaced by 0 after the loads)
   bool cin[2];
 #pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const int gx = gxa + b;
-    col[b] = (uint32_t)min(max(gx, 0), S - 1);
-    cin[b] = gx >= 0 && gx < S;
-  }
+  for (int b = 0; b < 2; ++b) cin[b] = gxa + b < S;  // gxa >= 0
   float x[R][2], x0[R][2];
   uint32_t cd[R];
-  // Loads first, all of them, so that one wait covers every load of the region.
-  uint32_t craw[R][2];
+  // Loads first, all of them, so that one wait covers every load of the
+  // region.  Addresses: the tile's first row as one wave-uniform base per
+  // array, a row's cells as a 32-bit offset from it (rows clamped into the
+  // grid).  A lane's column pair is one 8-byte load per field at
+  // min(gxa, S - 2) (gxa >= 0 is even): a pair that straddles the far edge
+  // (odd S, gxa = S - 1) reads (S - 2, S - 1) and takes column S - 1 from the
+  // second half; pairs past the edge read in-grid cells that are zeroed below.
+  // The pair's rule bytes are one 2-byte load (rule rows have an even
+  // stride Sc, so min(gxa, Sc - 2) is even; the straddling pair reads column
+  // S - 1 and the row's pad byte).
+  const int Sc = rm_code_stride(S);
+  const uint32_t pc = (uint32_t)min(gxa, S - 2), pcc = (uint32_t)min(gxa, Sc - 2);
+  const bool hi = gxa > S - 2;
+  GAS const char* sb = (GAS const char*)(src + (int64_t)gyb * S);  // gyb <= S - 1
+  GAS const char* xb = (GAS const char*)(x0g + (int64_t)gyb * S);
+  GAS const uint8_t* cb = code + (int64_t)gyb * Sc;
+  typedef float f2 __attribute__((ext_vector_type(2), aligned(4)));
+  uint32_t cw[R];  // rule bytes of the pair, first column low
 #pragma unroll
   for (int i = 0; i < R; ++i) {
-    const int64_t row = (int64_t)min(max(gy0 + i, 0), S - 1) * S;  // wave-uniform
-    // scalar row base + 32-bit byte offset: one VGPR per column, shared by all rows
-    GAS const char* sp = (GAS const char*)(src + row);
-    GAS const char* xp = (GAS const char*)(x0g + row);
-    GAS const uint8_t* cp = code + row;
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      x[i][b] = *(GAS const float*)(sp + 4u * col[b]);
-      x0[i][b] = *(GAS const float*)(xp + 4u * col[b]);
-      craw[i][b] = cp[col[b]];
-    }
+    const uint32_t rr = (uint32_t)(min(gy0 + i, S - 1) - gyb);  // gy0 >= gyb
+    const uint32_t eo = rr * (uint32_t)S + pc;
+    const f2 xv = *(GAS const f2*)(sb + 4u * eo);
+    const f2 x0v = *(GAS const f2*)(xb + 4u * eo);
+    x[i][0] = hi ? xv.y : xv.x;
+    x[i][1] = xv.y;
+    x0[i][0] = hi ? x0v.y : x0v.x;
+    x0[i][1] = x0v.y;
+    cw[i] = *(GAS const uint16_t*)(cb + (rr * (uint32_t)Sc + pcc));
   }
   // The rule byte (eqf_layout.h RuleBits) of each cell, as this mode's
   // selector; cells outside the grid (clamped loads) get none.
@@ -243,7 +252,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
       const bool inside = rin && cin[b];
       x[i][b] = inside ? x[i][b] : 0.0f;
       x0[i][b] = inside ? x0[i][b] : 0.0f;
-      const uint32_t c = craw[i][b];
+      const uint32_t c = (cw[i] >> (8 * b)) & 255u;
       uint32_t sel = (c & R_INTERIOR) ? (uint32_t)BS_UPD : 0u;
       if constexpr (MODE == M_ROW) sel |= (c >> R_ROW_SHIFT) & 3u;  // BS_NEG_LEFT / BS_NEG_RIGHT
       if constexpr (MODE == M_COL) {
@@ -270,7 +279,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
   const bool rules = __builtin_amdgcn_ballot_w64((anys & 0x0f0fu) != 0) != 0;
 
   int par = 0;
-  for (int k = 0; k < nsweeps; ++k) {
+  // nsweeps >= 1: a loop without an entry test keeps the x0 loads above in the
+  // first block (a skippable loop lets them sink below the other loads' wait)
+  int k = 0;
+  do {
 #pragma unroll
     for (int ph = 0; ph < 3; ++ph) {  // red, black, set_boundaries
       // the rule bytes are loop-invariant; keeping the compilers from hoisting
@@ -345,7 +357,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
         }
       }
     }
-  }
+  } while (++k < nsweeps);
 
   // (columns recomputed here rather than kept live through the sweeps)
   // Output tile: [gx0 + H, gx0 + H + TX) x [gyb + H, gyb + H + TY), from the
@@ -365,9 +377,16 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
     const int gy = gy0 + i;
     if (gy < ylo || gy >= yhi) continue;  // wave-uniform
     GAS char* dp = (GAS char*)(dst + (int64_t)gy * S);
+    if (cout[0] && cout[1]) {  // one 8-byte store; only the tile's edge lanes store singles
+      f2 v;
+      v.x = x[i][0];
+      v.y = x[i][1];
+      *(GAS f2*)(dp + 4u * (uint32_t)gxl) = v;
+    } else {
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
-      if (cout[b]) *(GAS float*)(dp + 4u * (uint32_t)(gxl + b)) = x[i][b];
+      for (int b = 0; b < 2; ++b)
+        if (cout[b]) *(GAS float*)(dp + 4u * (uint32_t)(gxl + b)) = x[i][b];
+    }
   }
 }
 

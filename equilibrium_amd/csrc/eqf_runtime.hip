@@ -270,7 +270,7 @@ static int scene_alloc(eqf_ctx* ctx, const eqf_params* p, eqf_scene** out, hipSt
     scene_free(s);
     return fail(EQF_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
-  const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)cyc_code_bytes(s->P, s->ns) : (size_t)s->S * s->S;
+  const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)cyc_code_bytes(s->P, s->ns) : (size_t)rm_code_bytes(s->S);
   auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
   const size_t b_fld = up(sizeof(float) * NFIELDS * (size_t)s->fstride);
   const size_t b_code = up(ncode), b_rowf = up((size_t)s->S);

@@ -138,8 +138,8 @@ static uint64_t estimate_bytes(const eqf_params& p) {
   if (p.gs_mode == EQF_GS_REDBLACK) {
     // fields + sources, rule bytes, ping-pong field, row flags, break columns, lists
     const uint64_t fs = (uint64_t)rm_field_floats(S), cells = (uint64_t)S * S;
-    return (NFIELDS + 3) * fs * 4 + cells + cells * 4 + (uint64_t)S * 5 + (uint64_t)N * 96 + sizeof(SceneDesc) +
-           8192;
+    return (NFIELDS + 3) * fs * 4 + (uint64_t)rm_code_bytes(S) + cells * 4 + (uint64_t)S * 5 + (uint64_t)N * 96 +
+           sizeof(SceneDesc) + 8192;
   }
   const uint64_t fs = (uint64_t)cyc_field_floats(S, P, ns);
   return (NFIELDS + 3) * fs * 4 + 3ull * (uint64_t)cyc_code_bytes(P, ns) + (uint64_t)S + (uint64_t)N * 96 +

@@ -123,7 +123,7 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
         r |= ((c & C_RWALL) ? R_NEG_SECOND : (c & C_LWALL) ? R_NEG_FIRST : 0u) << R_ROW_SHIFT;
         r |= ((c & C_UWALL) ? R_NEG_FIRST : (c & C_DWALL) ? R_NEG_SECOND : 0u) << R_COL_SHIFT;
       }
-      code[off_of(g, x, y)] = (uint8_t)r;
+      code[(int64_t)y * rm_code_stride(S) + x] = (uint8_t)r;
     }
     if (fill && g.layout == L_CYCLIC) {
       const int64_t o = cyc_code_word(x, y, g.P), mstride = cyc_code_bytes(g.P, g.ns) / 4;
@@ -168,9 +168,9 @@ __global__ void k_codes(Geo g, const uint8_t* __restrict__ m, const uint8_t* __r
     if (on) {
       const int k = atomicAdd(&cnt[2], 1);
       if (fill) lpas[k] = make_int2((int)off_of(g, x, y), (int)off_of(g, sx, sy));
-      if (fill && g.layout == L_ROWMAJOR) {  // row-major: e == y*S + x
+      if (fill && g.layout == L_ROWMAJOR) {
         const uint32_t r = y == 0 ? R_COPY_DOWN : y == S - 1 ? R_COPY_UP : x == 0 ? R_COPY_RIGHT : R_COPY_LEFT;
-        code[e] = (uint8_t)(r << R_PAS_SHIFT);
+        code[(int64_t)y * rm_code_stride(S) + x] = (uint8_t)(r << R_PAS_SHIFT);
       }
     }
   }
@@ -302,7 +302,7 @@ int eqf_dev_rebuild_bnd(eqf_scene* s) {
     s->bnd_cap = total;
   }
   {
-    const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)cyc_code_bytes(s->P, s->ns) : (size_t)n;
+    const size_t ncode = s->layout == L_CYCLIC ? 3 * (size_t)cyc_code_bytes(s->P, s->ns) : (size_t)rm_code_bytes(S);
     e = hipMemsetAsync(s->d_code, 0, ncode, s->stream);
   }
   if (e == hipSuccess) {  // the fill pass counts into the second, still zero, counters
