@@ -325,6 +325,42 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
             x[i][b] = ((cd[i] >> (8 * b)) & BS_UPD) ? nv : x[i][b];
           }
         }
+      } else if constexpr (MODE == M_PASSIVE) {
+        // Passive set_boundaries (fluid.rs:179-187) writes ring cells only,
+        // each from its interior neighbour (column 0 from column 1, column
+        // S-1 from S-2, row 0 from row 1, row S-1 from S-2, where the rule
+        // byte says its row / column holds a NoWall cell; corners have none),
+        // so only the tiles on the grid's edge have work, at known places.
+        if (bx == 0) {  // column 0: the first column of lane 0
+#pragma unroll
+          for (int i = 0; i < R; ++i) x[i][0] = (cd[i] & 15u) == BS_RIGHT ? x[i][1] : x[i][0];
+        }
+        if (bx == tiles_x - 1) {  // column S-1
+          if (S & 1) {  // the first column of its lane: its left neighbour is in the lane before
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+              const float l = from_prev_lane(x[i][1]);
+              x[i][0] = (cd[i] & 15u) == BS_LEFT ? l : x[i][0];
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < R; ++i) x[i][1] = ((cd[i] >> 8) & 15u) == BS_LEFT ? x[i][0] : x[i][1];
+          }
+        }
+        if (gy0 == 0) {  // row 0 (R >= 2)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) x[0][b] = ((cd[0] >> (8 * b)) & 15u) == BS_DOWN ? x[1][b] : x[0][b];
+        }
+        if (gy0 + R > S - 1) {  // row S-1 (only its cells carry BS_UP)
+#pragma unroll
+          for (int i = R - 1; i >= 0; --i) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
+              x[i][b] = ((cd[i] >> (8 * b)) & 15u) == BS_UP ? up : x[i][b];
+            }
+          }
+        }
       } else if (rules) {
         // set_boundaries (fluid.rs:141-188): writes only NoWall or ring
         // cells and reads only walls or interior cells, so in place is safe
