@@ -61,6 +61,14 @@ __device__ __forceinline__ float from_next_lane(float v) {
   asm volatile("" : "+v"(r));
   return r;
 }
+// Unpinned shifts for straight-line code with every lane active and no
+// select on the result: the compiler may fold them into the add that uses them.
+__device__ __forceinline__ float prev_lane_add(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float next_lane_add(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
+}
 
 // set_boundaries rule of one cell in one mode (k_rb_band): low nibble of the
 // per-cell byte, bit 4 = the cell is interior (updated by the sweeps).
@@ -306,8 +314,8 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
 #pragma unroll
           for (int i = 0; i < R; ++i) {
             const int b = (i + ph) & 1;
-            const float left = b ? x[i][0] : from_prev_lane(x[i][1]);
-            const float right = b ? from_next_lane(x[i][0]) : x[i][1];
+            const float left = b ? x[i][0] : prev_lane_add(x[i][1]);
+            const float right = b ? next_lane_add(x[i][0]) : x[i][1];
             const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
             const float down = i < R - 1 ? x[i + 1][b] : (b ? be.y : be.x);
             x[i][b] = (x0[i][b] + a * (((right + left) + down) + up)) * cr;
