@@ -283,6 +283,10 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
     allu &= cd[i];
     anys |= cd[i];
   }
+  // two rows' selectors per register (row 2k low half, 2k+1 high half)
+  uint32_t cp[R / 2];
+#pragma unroll
+  for (int k = 0; k < R / 2; ++k) cp[k] = cd[2 * k] | (cd[2 * k + 1] << 16);
   const bool full = __builtin_amdgcn_ballot_w64((allu & (BS_UPD | (BS_UPD << 8))) != (BS_UPD | (BS_UPD << 8))) == 0;
   const bool rules = __builtin_amdgcn_ballot_w64((anys & 0x0f0fu) != 0) != 0;
 
@@ -296,7 +300,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
       // the rule bytes are loop-invariant; keeping the compilers from hoisting
       // every per-cell lane mask out of the sweep loop (into scalar registers)
 #pragma unroll
-      for (int i = 0; i < R; ++i) asm volatile("" : "+v"(cd[i]));
+      for (int k = 0; k < R / 2; ++k) asm volatile("" : "+v"(cp[k]));
       float2 ab = make_float2(0.0f, 0.0f), be = make_float2(0.0f, 0.0f);  // rows above / below the band
       if (ph < 2 || MODE != M_ROW) {
         edg[par][0][0][w][lane] = x[0][0];
@@ -330,7 +334,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
             const float down = i < R - 1 ? x[i + 1][b] : (b ? be.y : be.x);
             float nv = (x0[i][b] + a * (((right + left) + down) + up)) * cr;
             asm volatile("" : "+v"(nv));  // a select, not a branch around the update
-            x[i][b] = ((cd[i] >> (8 * b)) & BS_UPD) ? nv : x[i][b];
+            x[i][b] = (cp[i >> 1] & (BS_UPD << (16 * (i & 1) + 8 * b))) ? nv : x[i][b];
           }
         }
       } else if constexpr (MODE == M_PASSIVE) {
@@ -341,23 +345,23 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
         // so only the tiles on the grid's edge have work, at known places.
         if (bx == 0) {  // column 0: the first column of lane 0
 #pragma unroll
-          for (int i = 0; i < R; ++i) x[i][0] = (cd[i] & 15u) == BS_RIGHT ? x[i][1] : x[i][0];
+          for (int i = 0; i < R; ++i) x[i][0] = ((cp[i >> 1] >> (16 * (i & 1))) & 15u) == BS_RIGHT ? x[i][1] : x[i][0];
         }
         if (bx == tiles_x - 1) {  // column S-1
           if (S & 1) {  // the first column of its lane: its left neighbour is in the lane before
 #pragma unroll
             for (int i = 0; i < R; ++i) {
               const float l = from_prev_lane(x[i][1]);
-              x[i][0] = (cd[i] & 15u) == BS_LEFT ? l : x[i][0];
+              x[i][0] = ((cp[i >> 1] >> (16 * (i & 1))) & 15u) == BS_LEFT ? l : x[i][0];
             }
           } else {
 #pragma unroll
-            for (int i = 0; i < R; ++i) x[i][1] = ((cd[i] >> 8) & 15u) == BS_LEFT ? x[i][0] : x[i][1];
+            for (int i = 0; i < R; ++i) x[i][1] = ((cp[i >> 1] >> (16 * (i & 1) + 8)) & 15u) == BS_LEFT ? x[i][0] : x[i][1];
           }
         }
         if (gy0 == 0) {  // row 0 (R >= 2)
 #pragma unroll
-          for (int b = 0; b < 2; ++b) x[0][b] = ((cd[0] >> (8 * b)) & 15u) == BS_DOWN ? x[1][b] : x[0][b];
+          for (int b = 0; b < 2; ++b) x[0][b] = ((cp[0] >> (8 * b)) & 15u) == BS_DOWN ? x[1][b] : x[0][b];
         }
         if (gy0 + R > S - 1) {  // row S-1 (only its cells carry BS_UP)
 #pragma unroll
@@ -365,7 +369,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
 #pragma unroll
             for (int b = 0; b < 2; ++b) {
               const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
-              x[i][b] = ((cd[i] >> (8 * b)) & 15u) == BS_UP ? up : x[i][b];
+              x[i][b] = ((cp[i >> 1] >> (16 * (i & 1) + 8 * b)) & 15u) == BS_UP ? up : x[i][b];
             }
           }
         }
@@ -376,7 +380,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
         for (int i = 0; i < R; ++i) {
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
-            const uint32_t sel = (cd[i] >> (8 * b)) & 15u;
+            const uint32_t sel = (cp[i >> 1] >> (16 * (i & 1) + 8 * b)) & 15u;
             float v = x[i][b];
             if constexpr (MODE == M_ROW || MODE == M_PASSIVE) {
               const float left = b ? x[i][0] : from_prev_lane(x[i][1]);
