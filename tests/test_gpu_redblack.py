@@ -22,10 +22,13 @@ def _case(label):
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("tsw", [1, 2, 4, 5, 6, 8])
-@pytest.mark.parametrize("N", [150, 330])
+@pytest.mark.parametrize("N", [150, 151, 330])
 def test_redblack_bit_exact_all_tilings(tile, tsw, N):
-    """N=150: two tiles per dimension (one at each edge); N=330: three or
-    more in most settings, so interior tiles with halos on both sides too."""
+    """N=150: two tiles per dimension (one at each edge); N=151: the same with
+    an odd size (the grid's last column is the first of a lane's pair, its
+    8-byte load straddles the edge and its store is a single); N=330: three
+    or more tiles in most settings, so interior tiles with halos on both sides
+    too."""
     _case(f"rb tile={tile} T={tsw} N={N}")
     sc = scenes.stam_scene(N, scenes.derive_seed(20, N), iters=7, frames=3, n_rects=4,
                            diffusion=1e-4, viscosity=1e-3)
