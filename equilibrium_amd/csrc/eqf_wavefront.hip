@@ -379,6 +379,11 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   }
 }
 
+// Non-temporal global load / store (the L2 line is the first to be evicted):
+// the one-pass phases stream whole fields that no other phase of the frame
+// finds in L2 again, so they leave the lines of the sweeps' re-reads alone.
+#define ntl(p) __builtin_nontemporal_load(p)
+#define nts(p, v) __builtin_nontemporal_store((v), (p))
 // The one-pass phases below walk each strip in blocks of 16 slots (P is a
 // multiple of 16, so a block never wraps): all loads of a block are issued
 // before any of its values is used, so a wave keeps ~50 loads in flight
@@ -439,8 +444,8 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
 #pragma unroll
     for (int k = 0; k < 17; ++k) {
       const int t = k < 16 ? tb + k : tn;
-      ub[k] = us[t * 64];
-      vb[k] = vs[t * 64];
+      ub[k] = ntl(&us[t * 64]);
+      vb[k] = ntl(&vs[t * 64]);
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) e[k] = eb[edge_index(tb + k, lane, rl, has_prev, has_next, N, P, S)];
@@ -460,8 +465,8 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
       const float vminus = from_prev_lane(vm, e[k]);  // lane 0: the value above the strip
       const float d = -0.5f * (uplus - uminus + vplus - vminus) / sf;
       if (row_ok && c0 < N) {
-        ds[t * 64] = d;
-        ps[t * 64] = 0.0f;
+        nts(&ds[t * 64], d);
+        nts(&ps[t * 64], 0.0f);
       }
     }
   }
@@ -497,13 +502,13 @@ __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__
     const int tn = tb + 8 == P ? 0 : tb + 8;
     const float pm1 = ps[tp * 64];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) pb[k] = ps[(k < 8 ? tb + k : tn) * 64];
+    for (int k = 0; k < 9; ++k) pb[k] = ntl(&ps[(k < 8 ? tb + k : tn) * 64]);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int t = tb + k;
       e[k] = eb[edge_index(t, lane, rl, has_prev, has_next, N, P, S)];
-      ub[k] = us[t * 64];
-      vb[k] = vs[t * 64];
+      ub[k] = ntl(&us[t * 64]);
+      vb[k] = ntl(&vs[t * 64]);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -516,8 +521,8 @@ __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__
       const float pdn = from_next_lane(lane == rl + 1 ? e[k] : pb[k + 1], e[k]);
       const float pup = from_prev_lane(pm, e[k]);
       if (row_ok && c0 < N) {
-        us[t * 64] = ub[k] - 0.5f * (pplus - pminus) * sf;
-        vs[t * 64] = vb[k] - 0.5f * (pdn - pup) * sf;
+        nts(&us[t * 64], ub[k] - 0.5f * (pplus - pminus) * sf);
+        nts(&vs[t * 64], vb[k] - 0.5f * (pdn - pup) * sf);
       }
     }
   }
@@ -640,8 +645,8 @@ __device__ __forceinline__ void advect_phase(const SceneDesc& sc, lf* __restrict
       const float g1 = two ? gather(d1in, tr, S, P, ns) : 0.0f;
       if (row_ok && c <= N && c <= trig) {
         const bool ring1 = at && c == 1;  // trigger in column 1: copy the ring cell (fluid.rs:421)
-        o0[t * 64] = ring1 ? left0 : g0;
-        if (two) o1[t * 64] = ring1 ? left1 : g1;
+        nts(&o0[t * 64], ring1 ? left0 : g0);
+        if (two) nts(&o1[t * 64], ring1 ? left1 : g1);
       }
     }
   }
@@ -663,12 +668,12 @@ __device__ __forceinline__ void add_sources_phase(const SceneDesc& sc, gf* f) {
   const int m = sc.has_src;  // only the sources that were set (adding 0 would turn -0 into +0)
   for (int e = threadIdx.x; e < n4; e += blockDim.x) {
     if (m & 1) {
-      const f4 a = sd[e];
-      fd[e] = fd[e] + a;
-      fs[e] = fs[e] + a;
+      const f4 a = ntl(&sd[e]);
+      nts(&fd[e], ntl(&fd[e]) + a);
+      nts(&fs[e], ntl(&fs[e]) + a);
     }
-    if (m & 2) fu[e] = fu[e] + su[e];
-    if (m & 4) fv[e] = fv[e] + sv[e];
+    if (m & 2) nts(&fu[e], ntl(&fu[e]) + ntl(&su[e]));
+    if (m & 4) nts(&fv[e], ntl(&fv[e]) + ntl(&sv[e]));
   }
   __syncthreads();
 }
@@ -684,11 +689,11 @@ __device__ __forceinline__ void copy_phase(const SceneDesc& sc, gf* __restrict__
   for (; e + 7 * stride < n4; e += 8 * stride) {
     f4 r[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) r[k] = s4[e + k * stride];
+    for (int k = 0; k < 8; ++k) r[k] = ntl(&s4[e + k * stride]);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) d4[e + k * stride] = r[k];
+    for (int k = 0; k < 8; ++k) nts(&d4[e + k * stride], r[k]);
   }
-  for (; e < n4; e += stride) d4[e] = s4[e];
+  for (; e < n4; e += stride) nts(&d4[e], ntl(&s4[e]));
   __syncthreads();
 }
 
