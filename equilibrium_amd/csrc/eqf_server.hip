@@ -259,10 +259,13 @@ static std::string lane_launch(eqf_server* sv, int li) {
     }
     if (uids != L.last_uids) {
       L.descs.resize(wf.size());
+      int maxP = 0;
+      for (Req* r : wf) maxP = std::max(maxP, r->scene->P);
       for (size_t i = 0; i < wf.size(); ++i) {
         L.descs[i] = wf[i]->scene->desc;
         L.descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
         L.descs[i].frame_mult = wf[i]->tick_frames;
+        L.descs[i].stream_l2 = eqf_stream_l2(maxP);
       }
       L.last_uids.clear();
       if ((int)wf.size() > L.desc_cap) {

@@ -221,8 +221,8 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   // values read only by stream 0's next sweep, about P steps later: its
   // interior blocks load and store non-temporally, so that L2 keeps the
   // lines the other streams are about to re-read (batched mix: 12.97 ->
-  // 12.32 ms per frame).
-  const bool nt_stream = m > 1 && jst == m - 1;
+  // 12.32 ms per frame); only where SceneDesc::stream_l2 says so.
+  const bool nt_stream = m > 1 && jst == m - 1 && sc.stream_l2 != 0;
   const int j = 64 * s + lane + 1;
   const bool valid = has_strip && j <= N;
   const int rl = min(63, N - 1 - 64 * s);  // last valid lane of the strip
@@ -379,11 +379,16 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   }
 }
 
-// Non-temporal global load / store (the L2 line is the first to be evicted):
-// the one-pass phases stream whole fields that no other phase of the frame
-// finds in L2 again, so they leave the lines of the sweeps' re-reads alone.
-#define ntl(p) __builtin_nontemporal_load(p)
-#define nts(p, v) __builtin_nontemporal_store((v), (p))
+// Global load / store, non-temporal (the L2 line is the first to be
+// evicted) where the enclosing phase's NT is set (SceneDesc::stream_l2):
+// the phases then stream their fields past L2 and leave the lines of the
+// sweeps' re-reads alone.
+#define ntl(p) (NT ? __builtin_nontemporal_load(p) : *(p))
+#define nts(p, v)                                    \
+  do {                                               \
+    if constexpr (NT) __builtin_nontemporal_store((v), (p)); \
+    else *(p) = (v);                                 \
+  } while (0)
 // The one-pass phases below walk each strip in blocks of 16 slots (P is a
 // multiple of 16, so a block never wraps): all loads of a block are issued
 // before any of its values is used, so a wave keeps ~50 loads in flight
@@ -411,6 +416,7 @@ __device__ __forceinline__ int edge_index(int t, int lane, int rl, bool has_prev
 
 // project, first half (fluid.rs:339-349): div and p = 0 on interior cells,
 // then set_boundaries(Passive) on both (fluid.rs:351-352).
+template <bool NT>
 __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__ u, gcf* __restrict__ v,
                                           gf* __restrict__ div, gf* __restrict__ p) {
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
@@ -475,6 +481,7 @@ __device__ __forceinline__ void div_phase(const SceneDesc& sc, gcf* __restrict__
 }
 
 // project, second half (fluid.rs:364-374)
+template <bool NT>
 __device__ __forceinline__ void grad_phase(const SceneDesc& sc, gf* __restrict__ u, gf* __restrict__ v,
                                            gcf* __restrict__ p) {
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
@@ -572,6 +579,7 @@ __device__ __forceinline__ float gather(gcf* d0, const Trace& r, int S, int P, i
 // buffer's previous content.  Pass A (one wave per strip) finds each row's
 // first trigger column and leaves it in LDS (`trig_lds`, one int per row);
 // pass B deals blocks of 8 slots to all waves.
+template <bool NT>
 __device__ __forceinline__ void advect_phase(const SceneDesc& sc, lf* __restrict__ trig_lds, int mode0,
                                              gf* __restrict__ d0out, gcf* __restrict__ d0in, int mode1,
                                              gf* __restrict__ d1out, gcf* __restrict__ d1in, gcf* __restrict__ u,
@@ -656,6 +664,7 @@ __device__ __forceinline__ void advect_phase(const SceneDesc& sc, lf* __restrict
 
 // Frame sources: add_density (fluid.rs:122-123) and add_velocity
 // (fluid.rs:129-130) applied to every cell, interior and ring.
+template <bool NT>
 __device__ __forceinline__ void add_sources_phase(const SceneDesc& sc, gf* f) {
   const int n4 = (int)(sc.fstride / 4);  // fstride is a multiple of 64
   gcf4* sd = (gcf4*)sc.src;
@@ -680,6 +689,7 @@ __device__ __forceinline__ void add_sources_phase(const SceneDesc& sc, gf* f) {
 
 // dst = src over the whole field (fstride is a multiple of 64 floats),
 // 8 float4 loads in flight per thread.
+template <bool NT>
 __device__ __forceinline__ void copy_phase(const SceneDesc& sc, gf* __restrict__ dst, gcf* __restrict__ src) {
   const int n4 = (int)(sc.fstride / 4);
   gcf4* s4 = (gcf4*)src;
@@ -714,7 +724,11 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
   gf* V = f + F_V * n;
   gf* U0 = f + F_U0 * n;
   gf* V0 = f + F_V0 * n;
-  if (sc.has_src) add_sources_phase(sc, f);
+  const bool big = sc.stream_l2 != 0;  // one-pass phases stream past L2
+  if (sc.has_src) {
+    if (big) add_sources_phase<true>(sc, f);
+    else add_sources_phase<false>(sc, f);
+  }
 #pragma nounroll
   for (int ph = 0; ph < 12; ++ph) {
     int kind = PK_GS, mode = M_PASSIVE, mode1 = M_PASSIVE, iters = 0;
@@ -741,13 +755,17 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
         else gs_wavefront<M_PASSIVE, D, B>(sc, H, x, y, a, cr, iters);
       }
     } else if (kind == PK_DIV) {
-      div_phase(sc, x, y, z, w);
+      if (big) div_phase<true>(sc, x, y, z, w);
+      else div_phase<false>(sc, x, y, z, w);
     } else if (kind == PK_GRAD) {
-      grad_phase(sc, x, y, z);
+      if (big) grad_phase<true>(sc, x, y, z);
+      else grad_phase<false>(sc, x, y, z);
     } else if (kind == PK_ADV) {
-      advect_phase(sc, H, mode, x, y, mode1, z, w, q, r);
+      if (big) advect_phase<true>(sc, H, mode, x, y, mode1, z, w, q, r);
+      else advect_phase<false>(sc, H, mode, x, y, mode1, z, w, q, r);
     } else {
-      copy_phase(sc, x, y);
+      if (big) copy_phase<true>(sc, x, y);
+      else copy_phase<false>(sc, x, y);
     }
   }
 }
