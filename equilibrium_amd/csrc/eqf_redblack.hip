@@ -172,7 +172,7 @@ __device__ __forceinline__ void coefs(const SceneDesc& d, int coef, float& a, fl
 // buffer and the tile is written to the other (ping-pong).  Corners are set
 // once after the lin_solve (nothing reads them during the sweeps).
 template <int MODE, int T, int NW, int R>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((NW == 8 && R <= 14) ? 6 : (NW == 8 && R <= 18) ? 4 : 1))) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((NW == 8 && R <= 10) ? 8 : (NW == 8 && R <= 14) ? 6 : (NW == 8 && R <= 18) ? 4 : 1))) k_rb_band(const RbScene* __restrict__ scs, const int2* __restrict__ tmap,
                                                      int nt, int src_role, int dst_role, int x0_role, int coef,
                                                      int nsweeps) {
   constexpr int H = 3 * T, W = 128, RH = NW * R;
@@ -742,7 +742,7 @@ struct RbGroup {
   int n = 0, maxN = 0, tiles = 0, iblocks = 0, cblocks = 0, bpp = 0, brc = 0, bp = 0;
   bool any_src = false;
   int diff_iters = 0, press_iters = 0;
-  int tile = 4, tsweeps = 5;  // band shape index (kBands) and sweeps per launch (untuned default)
+  int tile = 5, tsweeps = 5;  // band shape index (kBands) and sweeps per launch (untuned default)
   int tab_tile = -1, tab_T = -1;
   hipGraphExec_t graph = nullptr;
   int64_t graph_frames = 0;
@@ -760,7 +760,7 @@ struct RBWork {
 struct BandCfg {
   int nw, r;
 };
-static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}, {8, 18}, {8, 14}};
+static const BandCfg kBands[] = {{8, 16}, {8, 32}, {16, 16}, {8, 18}, {8, 14}, {8, 10}};
 
 static int band_tiles(int S, int tile, int T) {
   const BandCfg b = kBands[tile];
@@ -898,6 +898,7 @@ static void launch_gs_mode(hipStream_t st, const RbGroup& g, int src, int dst, i
   else if (g.tile == 2) launch_band_t<MODE, 16, 16>(st, g, src, dst, x0, coef, nsw);
   else if (g.tile == 3) launch_band_t<MODE, 8, 18>(st, g, src, dst, x0, coef, nsw);
   else if (g.tile == 4) launch_band_t<MODE, 8, 14>(st, g, src, dst, x0, coef, nsw);
+  else if (g.tile == 5) launch_band_t<MODE, 8, 10>(st, g, src, dst, x0, coef, nsw);
   else launch_band_t<MODE, 8, 16>(st, g, src, dst, x0, coef, nsw);
 }
 
@@ -1061,8 +1062,8 @@ void* rb_create(eqf_scene* s, std::string& err) {
     return nullptr;
   }
   w->one.scenes = {s};
-  w->one.tile = 0;  // the autotuner (equilibrium_amd/tune.py) may override these
-  w->one.tsweeps = 4;
+  w->one.tile = 5;  // the autotuner (equilibrium_amd/tune.py) may override these
+  w->one.tsweeps = 5;
   return w;
 }
 
@@ -1077,7 +1078,7 @@ void rb_destroy(eqf_scene* s) {
 }
 
 static bool tiling_ok(int tile, int tsweeps) {
-  return tile >= 0 && tile <= 4 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
+  return tile >= 0 && tile <= 5 && tsweeps >= 1 && tsweeps <= 8 && tsweeps != 3 && tsweeps != 7;
 }
 
 int rb_set_tiling(eqf_scene* s, int tile, int tsweeps) {

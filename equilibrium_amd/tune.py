@@ -21,7 +21,7 @@ import time
 from . import _abi
 
 # band shape index -> (waves per workgroup, rows per wave); regions are 128 columns wide
-TILES = {0: (8, 16), 1: (8, 32), 2: (16, 16), 3: (8, 18), 4: (8, 14)}
+TILES = {0: (8, 16), 1: (8, 32), 2: (16, 16), 3: (8, 18), 4: (8, 14), 5: (8, 10)}
 SWEEPS_PER_BLOCK = (2, 4, 5, 6, 8)
 CACHE = os.environ.get("EQF_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                        "tuning_cache.json"))

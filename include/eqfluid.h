@@ -143,8 +143,9 @@ int eqf_sync(eqf_scene *s);
 
 /* Red-black mode tiling (no counterpart in the reference): band shape index
  * (0: 8 waves x 16 rows, 1: 8 waves x 32 rows, 2: 16 waves x 16 rows, 3: 8
- * waves x 18 rows, 4: 8 waves x 14 rows; regions 128 columns wide) and
- * Gauss-Seidel sweeps per launch (1, 2, 4, 5, 6 or 8); 4 and 5 until set.
+ * waves x 18 rows, 4: 8 waves x 14 rows, 5: 8 waves x 10 rows; regions 128
+ * columns wide) and Gauss-Seidel sweeps per launch (1, 2, 4, 5, 6 or 8); 5
+ * and 5 until set.
  * Results do not depend on it.  Chosen by the autotuner
  * (equilibrium_amd/tune.py); the server's red-black groups use the default. */
 int eqf_scene_set_tuning(eqf_scene *s, int tile, int sweeps_per_block);

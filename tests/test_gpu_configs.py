@@ -137,7 +137,7 @@ def test_redblack_4096_bit_exact_and_fp64_error():
     dt, diff, visc = np.float32(sc["dt"]), np.float32(sc["diffusion"]), np.float32(sc["viscosity"])
     _case("N=4096 red-black: 1 frame vs f32 red-black oracle")
     g = gpu_scene(sc, _abi.GS_REDBLACK)
-    g.set_tuning(4, 5)   # band 8 waves x 14 rows, 5 sweeps per launch
+    g.set_tuning(5, 5)   # band 8 waves x 10 rows, 5 sweeps per launch (the default)
     g.step(1)
     g.sync()
     got1 = download_all(g)

@@ -20,7 +20,7 @@ def _case(label):
     print(f"[case] {label}", flush=True)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("tsw", [1, 2, 4, 5, 6, 8])
 @pytest.mark.parametrize("N", [150, 151, 330])
 def test_redblack_bit_exact_all_tilings(tile, tsw, N):
@@ -115,7 +115,7 @@ def test_redblack_batch_equals_solo_and_oracle():
         s.close()
         if sc["size"] <= 152:
             assert_state_equal(want[k], run_oracle(sc, order=oracle.RB), f"rb solo N={sc['size'] - 2}")
-    for tile, tsw in ((0, 5), (2, 8), (1, 2), (4, 6)):
+    for tile, tsw in ((0, 5), (2, 8), (1, 2), (4, 6), (5, 5)):
         _case(f"rb batch band {tile}:{tsw}")
         hs = [gpu_scene(sc, _abi.GS_REDBLACK) for sc in mix]
         b = _abi.Batch(hs)
