@@ -374,31 +374,22 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu((N
           }
         }
       } else if (rules) {
-        // set_boundaries (fluid.rs:141-188): writes only NoWall or ring
-        // cells and reads only walls or interior cells, so in place is safe
+        // set_boundaries AdjustRow / AdjustColumn (fluid.rs:141-178): writes
+        // only NoWall cells and reads only walls, so in place is safe
 #pragma unroll
         for (int i = 0; i < R; ++i) {
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
             const uint32_t sel = (cp[i >> 1] >> (16 * (i & 1) + 8 * b)) & 15u;
             float v = x[i][b];
-            if constexpr (MODE == M_ROW || MODE == M_PASSIVE) {
+            if constexpr (MODE == M_ROW) {
               const float left = b ? x[i][0] : from_prev_lane(x[i][1]);
               const float right = b ? from_next_lane(x[i][0]) : x[i][1];
-              if constexpr (MODE == M_ROW) {
-                v = sel == BS_NEG_RIGHT ? -right : (sel == BS_NEG_LEFT ? -left : v);
-              } else {
-                v = sel == BS_RIGHT ? right : (sel == BS_LEFT ? left : v);
-              }
-            }
-            if constexpr (MODE == M_COL || MODE == M_PASSIVE) {
+              v = sel == BS_NEG_RIGHT ? -right : (sel == BS_NEG_LEFT ? -left : v);
+            } else {
               const float up = i > 0 ? x[i - 1][b] : (b ? ab.y : ab.x);
               const float down = i < R - 1 ? x[i + 1][b] : (b ? be.y : be.x);
-              if constexpr (MODE == M_COL) {
-                v = sel == BS_NEG_UP ? -up : (sel == BS_NEG_DOWN ? -down : v);
-              } else {
-                v = sel == BS_DOWN ? down : (sel == BS_UP ? up : v);
-              }
+              v = sel == BS_NEG_UP ? -up : (sel == BS_NEG_DOWN ? -down : v);
             }
             x[i][b] = v;
           }
