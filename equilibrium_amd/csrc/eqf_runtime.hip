@@ -586,7 +586,7 @@ extern "C" int eqf_scene_set_tuning(eqf_scene* s, int tile, int sweeps_per_block
   if (!s) return fail(EQF_EINVAL, "scene is NULL");
   if (s->layout != L_ROWMAJOR) return fail(EQF_EINVAL, "tuning applies to red-black scenes");
   if (rb_set_tiling(s, tile, sweeps_per_block) != EQF_OK)
-    return fail(EQF_EINVAL, "tile must be 0..4 and sweeps_per_block 1, 2, 4, 5, 6 or 8");
+    return fail(EQF_EINVAL, "tile must be 0..5 and sweeps_per_block 1, 2, 4, 5, 6 or 8");
   return EQF_OK;
 }
 
@@ -715,7 +715,7 @@ extern "C" int eqf_batch_set_band(eqf_batch* b, int tile, int sweeps_per_block) 
   HIPCHK(hipSetDevice(b->ctx->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   if (rb_batch_set_tiling(b->rbg, tile, sweeps_per_block) != EQF_OK)
-    return fail(EQF_EINVAL, "tile must be 0..4 and sweeps 1, 2, 4, 5, 6 or 8");
+    return fail(EQF_EINVAL, "tile must be 0..5 and sweeps 1, 2, 4, 5, 6 or 8");
   return EQF_OK;
 }
 
