@@ -48,7 +48,8 @@ def b_alg(iters):
 
 def profiled_traffic(kernel, shape=None):
     """HBM bytes per launch measured by the committed rocprofv3 PMC runs, for
-    the launch shape ("max_waves:order") the run used; None when no profile
+    the launch shape (batch: "max_waves:order:schedule", red-black:
+    "band:sweeps") the run used; None when no profile
     was taken at that shape.  Returns (bytes, profile entry)."""
     try:
         with open(os.path.join(REPO, "profiles", "traffic.json")) as f:
@@ -98,6 +99,9 @@ def device_sync():
         pass
 
 
+SCHED_NAMES = {0: "auto", 1: "wide", 2: "ring"}  # eqf_batch_set_schedule
+
+
 def make_scenes(scs):
     hs = []
     for sc in scs:
@@ -114,13 +118,14 @@ def bench_batch(scs, steps, warmup, tuning=None):
 
     hs = make_scenes(scs)
     if tuning:
-        (w, o), timings = tuning, {}
-        log(f"batch: max waves {w}, order {o} (given)")
+        (w, o, sch), timings = (tuning + (_abi.SCHED_AUTO,))[:3], {}
+        log(f"batch: max waves {w}, order {o}, schedule {SCHED_NAMES[sch]} (given)")
     else:
-        w, o, timings = tune.tune_batch(hs)
-        log(f"batch: autotuned max waves {w}, order {o}: {timings}")
+        w, o, sch, timings = tune.tune_batch(hs)
+        log(f"batch: autotuned max waves {w}, order {o}, schedule {SCHED_NAMES[sch]}: {timings}")
     b = _abi.Batch(hs)
     b.set_tuning(w, o)
+    b.set_schedule(sch)
     if warmup:
         b.step(warmup)
         b.sync()
@@ -140,7 +145,8 @@ def bench_batch(scs, steps, warmup, tuning=None):
         ("density", "scratch", "vx", "vy", "vx0", "vy0"))}) for i in sorted(picks.values())], warmup + steps)
     for h in hs:
         h.close()
-    return wall, ev_ms, {"max_waves": w, "order": "largest first" if o == 0 else "given", "trial_ms": timings}
+    return wall, ev_ms, {"max_waves": w, "order": "largest first" if o == 0 else "given",
+                         "schedule": SCHED_NAMES[sch], "shape": f"{w}:{o}:{sch}", "trial_ms": timings}
 
 
 def check_batch_exact(samples, frames, threads):
@@ -601,7 +607,8 @@ def main():
     ap.add_argument("--rb-frames", type=int, default=10)
     ap.add_argument("--rb-check-frames", type=int, default=20)
     ap.add_argument("--batch-tuning", type=str, default="",
-                    help="max_waves:order for the batch instead of autotuning (e.g. 16:0)")
+                    help="max_waves:order[:schedule] for the batch instead of autotuning (e.g. 16:0:2; "
+                         "schedule 0 auto, 1 wide, 2 ring)")
     ap.add_argument("--rb-tiling", type=str, default="",
                     help="band:sweeps for the N=4096 red-black grid instead of autotuning (e.g. 0:5)")
     ap.add_argument("--rb-batch-band", type=str, default="",
@@ -686,7 +693,7 @@ def main():
                 "launch_ms": launch_ms,
             },
         })
-        shape = f"{tuning['max_waves']}:{0 if tuning['order'] == 'largest first' else 1}"
+        shape = tuning["shape"]
         traffic, ent = profiled_traffic("eqf_frames_cyclic", shape)
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_frac"] = (traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None

@@ -19,6 +19,7 @@ DENSITY, SCRATCH, VX, VY, VX0, VY0 = range(6)
 DEFAULT_WALL, NO_WALL = 0, 1
 GS_WAVEFRONT, GS_REDBLACK = 0, 1
 BATCH_LARGEST_FIRST, BATCH_GIVEN_ORDER = 0, 1
+SCHED_AUTO, SCHED_WIDE, SCHED_RING = 0, 1, 2
 
 
 class EqfError(RuntimeError):
@@ -139,6 +140,7 @@ def lib():
             "eqf_batch_destroy": (I, [P]),
             "eqf_batch_last_ms": (I, [P, ctypes.POINTER(ctypes.c_double)]),
             "eqf_batch_set_tuning": (I, [P, I, I]),
+            "eqf_batch_set_schedule": (I, [P, I]),
             "eqf_batch_set_band": (I, [P, I, I]),
             "eqf_server_create": (I, [P, ctypes.POINTER(ServerConfig), PP]),
             "eqf_server_submit": (I, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
@@ -390,6 +392,10 @@ class Batch:
 
     def set_tuning(self, max_waves, order=BATCH_LARGEST_FIRST):
         check(lib().eqf_batch_set_tuning(self.h, max_waves, order))
+
+    def set_schedule(self, sched):
+        """Wavefront schedule of a wavefront-mode batch (SCHED_AUTO, SCHED_WIDE, SCHED_RING)."""
+        check(lib().eqf_batch_set_schedule(self.h, sched))
 
     def set_band(self, tile, sweeps_per_block):
         """Band shape and sweeps per launch of a red-black batch."""

@@ -65,12 +65,17 @@ int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
 // (strip count), then 2, 1, 0 for each halving.
 int eqf_batch_prio(int ns, int maxns);
 int eqf_stream_l2(int maxP);
-// Waves a wavefront-mode scene uses: its strips times the sweep streams per
-// strip (up to 16 waves, at most P/64 streams).
-int eqf_scene_waves(int ns, int P);
-// waves / lds_floats: the largest eqf_scene_waves() and cyc_lds_floats() of the scenes.
+// Waves a wavefront-mode scene uses under schedule `sched` (GsSched): its
+// strips times the sweep streams per strip (up to 16 waves, at most
+// P / stream lag streams).
+int eqf_scene_waves(int ns, int P, int sched);
+// The schedule (GsSched) of a launch of n scenes whose largest has maxns
+// strips, with at most max_waves waves per workgroup.
+int eqf_pick_sched(int n, int maxns, int max_waves);
+// waves / lds_floats: the largest eqf_scene_waves() and cyc_lds_floats() of
+// the scenes under schedule `sched`.
 hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int waves, int lds_floats, int nframes,
-                               hipStream_t st);
+                               int sched, hipStream_t st);
 int eqf_internal_error(int code, const std::string& msg);
 // Device address of eqf_render_rgba_noalloc's fixed scratch on `device`
 // (eqf_render.hip), looked up once per device at context creation.

@@ -101,18 +101,19 @@ struct SceneDesc {
                          // L2 again, or a launch shared with such scenes)
 };
 
-// Exact-mode wavefront schedule: prefetch depth (steps) and steps between
-// workgroup barriers.  Compile-time so that every access offset is an
-// immediate.
-#ifndef EQF_GS_D
-#define EQF_GS_D 8
-#endif
-#ifndef EQF_GS_B
-#define EQF_GS_B 16
-#endif
+// Exact-mode wavefront schedules (compile-time per kernel, so that every
+// access offset is an immediate): prefetch depth GS_D steps, B steps between
+// workgroup barriers, and whether x0 goes from sweep stream to sweep stream
+// through LDS (cyc_x0_ring).
+//   GS_WIDE: B = 16, x0 read from HBM by every stream (fewest instructions
+//            per step: lone scenes and small launches, issue-bound)
+//   GS_RING: B = 8, x0 handed on in LDS (a shorter stream lag and fewer HBM
+//            bytes: large launches, bound by the memory system)
+enum GsSched { GS_WIDE = 0, GS_RING = 1 };
+constexpr int GS_D = 8;
+EQF_HD constexpr int gs_block(int sched) { return sched == GS_RING ? 8 : 16; }
 
-
-EQF_HD inline int cyc_period(int N) { return (N + 15) & ~15; }
+EQF_HD constexpr int cyc_period(int N) { return (N + 15) & ~15; }
 
 // Wavefront lags for blocks of B steps (B divides 16) and prefetch depth
 // D <= B, both multiples of B.  A value handed from one wave to another must be read
@@ -136,13 +137,50 @@ EQF_HD constexpr int cyc_stream_lag(int B, int D) { return (cyc_strip_lag(B, D) 
 // hand off (all such lanes of a store write the same float: no bank conflict).
 // Rows are P + 1 floats apart, so that lane 63's edge read (the strip below)
 // and the other lanes' shared read (the strip above) fall in different banks.
-EQF_HD inline int cyc_lds_row(int P) { return P + 1; }
+EQF_HD constexpr int cyc_lds_row(int P) { return P + 1; }
+EQF_HD constexpr int gs_strip_lag(int sched) { return cyc_strip_lag(gs_block(sched), GS_D); }
+EQF_HD constexpr int gs_stream_lag(int sched) { return cyc_stream_lag(gs_block(sched), GS_D); }
+// Sweep streams per strip of a scene run by `waves` waves (lin_solves with
+// fewer sweeps than that use fewer).
+EQF_HD constexpr int cyc_streams(int waves, int ns, int P, int sched) {
+  int m = waves / (ns > 0 ? ns : 1);
+  const int mp = P / gs_stream_lag(sched);
+  m = m < mp ? m : mp;
+  return m > 1 ? m : 1;
+}
+// x0 (a lin_solve's right-hand side, the same in every sweep) goes from each
+// sweep stream of a strip to the next through a ring of slot rows in LDS
+// instead of being read from HBM again: stream j stores the 64 values of its
+// step tau at entry tau mod R, and stream j+1 loads them D steps before its
+// own step tau (its first D entries in its first block), LAG - D >= B steps,
+// so a barrier, later.  Stream j overwrites the entry at its step tau + R,
+// which with R >= LAG + B lies in a later block than either load.
+EQF_HD constexpr int cyc_x0_ring(int B, int D) { return cyc_stream_lag(B, D) + B; }
+// LDS layout (floats): the hand-off rows and their sink, then (GS_RING, at
+// cyc_x0_base) a sink of B slot rows for x0 stores nobody reads and one ring
+// of R slot rows per pair of consecutive streams of each strip.  At most
+// 39448 floats (154 KiB, N = 241 with 16 waves) of the 160 KiB of a CU.
+EQF_HD constexpr int cyc_x0_base(int ns, int P) { return 2 * ns * cyc_lds_row(P) + 16; }
 // The one-pass phases reuse the space for the advect trigger columns (one int
 // per row: 64 per strip).
-EQF_HD inline int cyc_lds_floats(int ns, int P) {
-  const int h = 2 * ns * cyc_lds_row(P) + 16;
+EQF_HD constexpr int cyc_lds_floats(int ns, int P, int waves, int sched) {
+  int h = cyc_x0_base(ns, P);
+  if (sched == GS_RING) {
+    const int B = gs_block(GS_RING);
+    h += B * 64 + ns * (cyc_streams(waves, ns, P, sched) - 1) * cyc_x0_ring(B, GS_D) * 64;
+  }
   return h > 64 * ns ? h : 64 * ns;
 }
+// Every exact-mode launch (N up to 1024, 1..16 waves, either schedule) fits
+// the 160 KiB of LDS of a CU.
+constexpr bool cyc_lds_fits() {
+  for (int sched = GS_WIDE; sched <= GS_RING; ++sched)
+    for (int w = 1; w <= 16; ++w)
+      for (int n = 1; n <= 1024; ++n)
+        if (cyc_lds_floats((n + 63) / 64, cyc_period(n), w, sched) > 160 * 1024 / 4) return false;
+  return true;
+}
+static_assert(cyc_lds_fits(), "an exact-mode launch would need more LDS than a CU has");
 
 // Ring offset inside a cyclic field.
 EQF_HD inline int64_t cyc_ring(int P, int ns) { return (int64_t)ns * P * 64; }

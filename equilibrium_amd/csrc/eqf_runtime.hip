@@ -19,7 +19,7 @@
 
 using namespace eqf;
 
-template <int D, int B, int MAXT>
+template <int D, int B, bool XR, int MAXT>
 __global__ void eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes);
 
 
@@ -554,10 +554,8 @@ extern "C" int eqf_download(eqf_scene* s, int field, float* host) {
   return download_rm(s, field_ptr(s, field), host);
 }
 
-int eqf_scene_waves(int ns, int P) {
-  const int lag = cyc_stream_lag(EQF_GS_B, EQF_GS_D);
-  const int m = std::max(1, std::min(16 / std::max(ns, 1), P / lag));  // sweep streams per strip
-  return std::max(1, std::min(16, ns * m));
+int eqf_scene_waves(int ns, int P, int sched) {
+  return std::max(1, std::min(16, ns * cyc_streams(16, ns, P, sched)));
 }
 
 extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
@@ -573,10 +571,12 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
     rc = rb_step(s, nframes, err);
     return rc ? fail(rc, err) : EQF_OK;
   }
-  const int W = eqf_scene_waves(s->ns, s->P);
+  // a lone scene is instruction-issue-bound: the schedule with the fewest
+  // instructions per step
+  const int W = eqf_scene_waves(s->ns, s->P, GS_WIDE);
   while (nframes > 0) {
     const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
-    HIPCHK(eqf_internal_launch(s->d_desc, 1, W, cyc_lds_floats(s->ns, s->P), nf, s->stream));
+    HIPCHK(eqf_internal_launch(s->d_desc, 1, W, cyc_lds_floats(s->ns, s->P, W, GS_WIDE), nf, GS_WIDE, s->stream));
     nframes -= nf;
   }
   return EQF_OK;
@@ -606,6 +606,8 @@ struct eqf_batch {
   int W = 1;
   int max_waves = 16;  // tuning: waves per workgroup at most (>= the largest strip count)
   int order = EQF_BATCH_LARGEST_FIRST;
+  int sched_req = EQF_SCHED_AUTO;  // tuning: wavefront schedule (eqf_batch_set_schedule)
+  int sched = GS_WIDE;             // the schedule in use (GsSched)
   void* rbg = nullptr;  // red-black batch: the launch group over all scenes (eqf_redblack.hip)
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -630,11 +632,13 @@ static int batch_layout(eqf_batch* b) {
   std::vector<SceneDesc> descs(n);
   int maxns = 1, maxw = 1;
   for (int i = 0; i < n; ++i) {
-    const eqf_scene* s = b->scenes[i];
-    descs[i] = s->desc;
-    maxns = std::max(maxns, s->ns);
-    maxw = std::max(maxw, eqf_scene_waves(s->ns, s->P));
+    descs[i] = b->scenes[i]->desc;
+    maxns = std::max(maxns, b->scenes[i]->ns);
   }
+  b->sched = b->sched_req == EQF_SCHED_RING   ? GS_RING
+             : b->sched_req == EQF_SCHED_WIDE ? GS_WIDE
+                                              : eqf_pick_sched(n, maxns, b->max_waves);
+  for (int i = 0; i < n; ++i) maxw = std::max(maxw, eqf_scene_waves(b->scenes[i]->ns, b->scenes[i]->P, b->sched));
   if (b->max_waves < maxns) return fail(EQF_EINVAL, "max_waves below the largest scene's strip count");
   int maxP = 0;
   for (int i = 0; i < n; ++i) maxP = std::max(maxP, b->scenes[i]->P);
@@ -719,6 +723,27 @@ extern "C" int eqf_batch_set_band(eqf_batch* b, int tile, int sweeps_per_block) 
   return EQF_OK;
 }
 
+extern "C" int eqf_batch_set_schedule(eqf_batch* b, int sched) {
+  if (!b) return fail(EQF_EINVAL, "batch is NULL");
+  if (b->rbg) return fail(EQF_EINVAL, "red-black batches have no wavefront schedule");
+  if (sched != EQF_SCHED_AUTO && sched != EQF_SCHED_WIDE && sched != EQF_SCHED_RING)
+    return fail(EQF_EINVAL, "sched must be EQF_SCHED_AUTO, EQF_SCHED_WIDE or EQF_SCHED_RING");
+  HIPCHK(hipSetDevice(b->ctx->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const int prev = b->sched_req;
+  b->sched_req = sched;
+  int rc = batch_layout(b);
+  if (rc) {  // keep the previous schedule
+    b->sched_req = prev;
+    (void)batch_layout(b);
+    return rc;
+  }
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);  // the graph holds the launch
+  b->gexec = nullptr;
+  b->graph_frames = 0;
+  return EQF_OK;
+}
+
 extern "C" int eqf_batch_set_tuning(eqf_batch* b, int max_waves, int order) {
   if (!b) return fail(EQF_EINVAL, "batch is NULL");
   if (b->rbg) return fail(EQF_EINVAL, "red-black batches are tuned with eqf_batch_set_band");
@@ -742,15 +767,38 @@ extern "C" int eqf_batch_set_tuning(eqf_batch* b, int max_waves, int order) {
   return EQF_OK;
 }
 
-hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int W, int lds_floats, int nframes,
+hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int W, int lds_floats, int nframes, int sched,
                                hipStream_t st) {
   const size_t lds = sizeof(float) * (size_t)lds_floats;
-  if (W <= 8)
-    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
-  else
-    hipLaunchKernelGGL((eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
-                       nframes);
+  constexpr int BW = gs_block(GS_WIDE), BR = gs_block(GS_RING);
+  if (sched == GS_RING) {
+    if (W <= 8)
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BR, true, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
+    else
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BR, true, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
+  } else {
+    if (W <= 8)
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BW, false, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
+    else
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BW, false, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
+                         nframes);
+  }
   return hipGetLastError();
+}
+
+// Wavefront schedule of a launch of n scenes whose largest has maxns strips,
+// with at most max_waves waves per workgroup.  GS_RING cuts the HBM bytes of
+// scenes with several sweep streams per strip but runs ~20% more
+// instructions per step: it pays where the memory system, not instruction
+// issue, sets the frame time, as with many scenes of 5 to 8 strips (N
+// 257..512) that get two or more streams each; a largest scene with one
+// stream gains nothing from the ring.  Measured (ms per frame, 20 sweeps,
+// wide / ring): 16 waves: 121 scenes N=512 11.03 / 10.50, the 512-scene mix
+// 12.24 / 11.93, 32 scenes N=512 7.80 / 8.19, one N=512 7.43 / 7.80, 130
+// N=256 2.52 / 2.74, 391 scenes N <= 256 3.10 / 3.07-3.11; 8 waves (one
+// stream per strip at N=512): 121 scenes N=512 12.12 / 14.17.
+int eqf_pick_sched(int n, int maxns, int max_waves) {
+  return (n >= 64 && maxns >= 5 && max_waves / maxns >= 2) ? GS_RING : GS_WIDE;
 }
 
 // Non-temporal streaming (SceneDesc::stream_l2) for the scenes of a launch
@@ -782,8 +830,9 @@ int eqf_internal_check_params(const eqf_params* p) { return check_params(p); }
 
 static hipError_t batch_launch(eqf_batch* b, int nframes_per_launch) {
   int lds = 0;
-  for (auto* s : b->scenes) lds = std::max(lds, cyc_lds_floats(s->ns, s->P));
-  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), b->W, lds, nframes_per_launch, b->stream);
+  for (auto* s : b->scenes) lds = std::max(lds, cyc_lds_floats(s->ns, s->P, b->W, b->sched));
+  return eqf_internal_launch(b->d_descs, (int)b->scenes.size(), b->W, lds, nframes_per_launch, b->sched,
+                             b->stream);
 }
 
 extern "C" int eqf_batch_step(eqf_batch* b, int64_t nframes) {

@@ -218,14 +218,14 @@ static void lane_tick_done(eqf_server* sv, Lane& L, double now) {
 // Dependent wavefront steps of one frame of a scene: its lin_solves' chains
 // (sweeps over m streams per strip, strip and stream fill); the one-pass
 // phases add a few passes over the strip.
-static double frame_chain(const eqf_scene* s) {
-  const int W = eqf_scene_waves(s->ns, s->P);
-  const int lag = cyc_stream_lag(EQF_GS_B, EQF_GS_D);
-  const int m = std::max(1, std::min(W / std::max(s->ns, 1), s->P / lag));
+static double frame_chain(const eqf_scene* s, int sched) {
+  const int W = eqf_scene_waves(s->ns, s->P, sched);
+  const int lag = gs_stream_lag(sched);
+  const int m = cyc_streams(W, s->ns, s->P, sched);
   auto solve = [&](int it) {
     if (it <= 0) return 0.0;
     const int mm = std::min(m, it);
-    return (double)((it + mm - 1) / mm) * s->P + (double)(s->ns - 1) * cyc_strip_lag(EQF_GS_B, EQF_GS_D) +
+    return (double)((it + mm - 1) / mm) * s->P + (double)(s->ns - 1) * gs_strip_lag(sched) +
            (double)(mm - 1) * lag + 64.0;
   };
   return 3.0 * solve(s->desc.diff_iters) + 2.0 * solve(s->desc.press_iters) + 4.0 * s->P;
@@ -243,16 +243,17 @@ static std::string lane_launch(eqf_server* sv, int li) {
     std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
     int maxns = 1, lds = 0, waves = 1;
     double chain_max = 0;
+    for (Req* r : wf) maxns = std::max(maxns, r->scene->ns);
+    const int sched = eqf_pick_sched((int)wf.size(), maxns, 16);
     for (Req* r : wf) {
-      maxns = std::max(maxns, r->scene->ns);
-      lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P));
-      waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P));
-      chain_max = std::max(chain_max, frame_chain(r->scene));
+      waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P, sched));
+      chain_max = std::max(chain_max, frame_chain(r->scene, sched));
     }
+    for (Req* r : wf) lds = std::max(lds, cyc_lds_floats(r->scene->ns, r->scene->P, waves, sched));
     std::vector<uint64_t> uids(2 * wf.size());
     for (size_t i = 0; i < wf.size(); ++i) {
       Req* r = wf[i];
-      const int64_t k = (int64_t)(chain_max / frame_chain(r->scene));
+      const int64_t k = (int64_t)(chain_max / frame_chain(r->scene, sched));
       r->tick_frames = (int)std::max<int64_t>(1, std::min<int64_t>({k, 16, r->nframes - r->frames_done}));
       uids[2 * i] = r->scene->uid;
       uids[2 * i + 1] = (uint64_t)r->tick_frames;
@@ -282,7 +283,7 @@ static std::string lane_launch(eqf_server* sv, int li) {
         e = hipMemcpyAsync(L.d_descs, L.descs.data(), sizeof(SceneDesc) * wf.size(), hipMemcpyHostToDevice, L.st);
       if (e == hipSuccess) L.last_uids = uids;
     }
-    if (e == hipSuccess) e = eqf_internal_launch(L.d_descs, (int)wf.size(), waves, lds, 1, L.st);
+    if (e == hipSuccess) e = eqf_internal_launch(L.d_descs, (int)wf.size(), waves, lds, 1, sched, L.st);
     if (e != hipSuccess) err = hipGetErrorString(e);
   } else {
     std::map<std::pair<int, int>, std::vector<eqf_scene*>> by_iters;

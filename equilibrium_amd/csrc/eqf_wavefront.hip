@@ -180,7 +180,7 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 // sweep reads such a ring value as `xself`, the previous sweep's value of the
 // cell being updated, and the ring itself is written once after the last
 // sweep by the list pass.
-template <int MODE, int D, int B>
+template <int MODE, int D, int B, bool XR>
 __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict__ H, gf* __restrict__ X,
                                              gcf* __restrict__ X0, float a, float cr, int iters) {
   static_assert((B == 16 || B == 8) && D <= B, "blocks of 8 or 16 steps (within one selector word), at least "
@@ -223,6 +223,19 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   // lines the other streams are about to re-read (batched mix: 12.97 ->
   // 12.32 ms per frame); only where SceneDesc::stream_l2 says so.
   const bool nt_stream = m > 1 && jst == m - 1 && sc.stream_l2 != 0;
+  // x0 hand-off ring (cyc_x0_ring): stream jst > 0 loads x0 from ring
+  // s*(m-1) + jst-1 in LDS instead of HBM; every stream but the last stores
+  // the x0 of each step into its own ring s*(m-1) + jst (the others into a
+  // sink that nobody reads).  Ring entries are slot rows indexed by the
+  // stream-local step modulo R, so writer and reader agree on them.
+  constexpr int R = cyc_x0_ring(B, D);
+  static_assert(!XR || (R % B == 0 && R >= LAG + B && LAG - D >= B), "x0 ring too short or read too early");
+  const bool rin = XR && has_strip && jst > 0;
+  const bool rout = XR && has_strip && jst < m - 1;
+  const int xsink = cyc_x0_base(ns, P);
+  const int xring = xsink + B * 64 + (s * (m - 1) + jst) * R * 64;  // this stream's ring (if rout)
+  const int xo_ring = rout ? xring : -1;           // < 0: stores go to the sink
+  const int xi_ring = rin ? xring - R * 64 : -1;  // < 0: x0 from HBM
   const int j = 64 * s + lane + 1;
   const bool valid = has_strip && j <= N;
   const int rl = min(63, N - 1 - 64 * s);  // last valid lane of the strip
@@ -308,6 +321,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   int tau = 0;       // strip-local step of the block's first step
   int tb = 0;        // its slot (uniform)
   int tbp = P - B;   // previous block's slot
+  int rbk = 0;       // x0 ring entry of the block's first step (tau mod R)
 
   float res_prev = 0.f, xl_prev = 0.f, xu_prev = 0.f, xd_prev = 0.f, xself = 0.f;
   uint32_t code_prev = 0;
@@ -324,7 +338,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
 #pragma unroll
     for (int i = 0; i < D; ++i) {  // prefetch the first D steps (slot i)
       fxs[i] = ldf(rX, vl + (i + 1) * 256, sb);
-      fx0[i] = ldf(rX0, vl + i * 256, sb);
+      fx0[i] = rin ? H[xi_ring + i * 64 + lane] : ldf(rX0, vl + i * 256, sb);
       fe[i] = H[e_row + i];
       fe2[i] = H[e_row + i];
     }
@@ -335,12 +349,48 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
 #pragma nounroll
     while (blk < nblk) {
       if (tau >= 64 && tau + B <= itP) {
+        // interior blocks, by where x0 comes from (XS: 0 HBM, 1 the ring)
+        // and whether it is handed on (XW)
         if (nt_stream) {
 #pragma nounroll
           do {
             {
               constexpr bool EDGE = false;
-              constexpr int POL = CP_NT;
+              constexpr int POL = CP_NT, XS = XR ? 1 : 0;
+              constexpr bool XW = false;
+#include "eqf_gs_block.inc"
+            }
+            ++blk;
+          } while (tau + B <= itP);
+        } else if (XR && rin && rout) {
+#pragma nounroll
+          do {
+            {
+              constexpr bool EDGE = false;
+              constexpr int POL = 0, XS = 1;
+              constexpr bool XW = true;
+#include "eqf_gs_block.inc"
+            }
+            ++blk;
+          } while (tau + B <= itP);
+        } else if (XR && rin) {
+#pragma nounroll
+          do {
+            {
+              constexpr bool EDGE = false;
+              constexpr int POL = 0, XS = 1;
+              constexpr bool XW = false;
+#include "eqf_gs_block.inc"
+            }
+            ++blk;
+          } while (tau + B <= itP);
+        } else if (XR && rout) {
+#pragma nounroll
+          do {
+            {
+              constexpr bool EDGE = false;
+              constexpr int POL = 0, XS = 0;
+              constexpr bool XW = true;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -350,7 +400,8 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
           do {
             {
               constexpr bool EDGE = false;
-              constexpr int POL = 0;
+              constexpr int POL = 0, XS = 0;
+              constexpr bool XW = false;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -358,8 +409,10 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
         }
       } else if (tau < itP + 64) {
         {
+          // both x0 sources, selected per wave; stores to the ring or the sink
           constexpr bool EDGE = true;
-          constexpr int POL = 0;
+          constexpr int POL = 0, XS = XR ? 2 : 0;
+          constexpr bool XW = XR;
 #include "eqf_gs_block.inc"
         }
         ++blk;
@@ -714,7 +767,7 @@ enum PhaseKind { PK_GS, PK_DIV, PK_GRAD, PK_ADV, PK_COPY };
 // divergence of the first projection, U0/V0 as those of the second.  The
 // phases run from a loop so that each phase kernel body exists once in the
 // code object.
-template <int D, int B>
+template <int D, int B, bool XR>
 __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
   gf* f = (gf*)sc.fld;
   const int n = (int)sc.fstride;
@@ -750,9 +803,9 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
     }
     if (kind == PK_GS) {
       if (iters > 0) {
-        if (mode == M_ROW) gs_wavefront<M_ROW, D, B>(sc, H, x, y, a, cr, iters);
-        else if (mode == M_COL) gs_wavefront<M_COL, D, B>(sc, H, x, y, a, cr, iters);
-        else gs_wavefront<M_PASSIVE, D, B>(sc, H, x, y, a, cr, iters);
+        if (mode == M_ROW) gs_wavefront<M_ROW, D, B, XR>(sc, H, x, y, a, cr, iters);
+        else if (mode == M_COL) gs_wavefront<M_COL, D, B, XR>(sc, H, x, y, a, cr, iters);
+        else gs_wavefront<M_PASSIVE, D, B, XR>(sc, H, x, y, a, cr, iters);
       }
     } else if (kind == PK_DIV) {
       if (big) div_phase<true>(sc, x, y, z, w);
@@ -772,10 +825,11 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
 
 }  // namespace eqf
 
-// grid = number of scenes, block = 64 * waves; scene b is descs[b].
-template <int D, int B, int MAXT>
+// grid = number of scenes, block = 64 * waves; scene b is descs[b].  D, B
+// and XR are the launch's schedule (GsSched).
+template <int D, int B, bool XR, int MAXT>
 __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes) {
-  extern __shared__ float eqf_lds[];  // cyc_lds_floats(ns, P) floats, sized by the launch
+  extern __shared__ float eqf_lds[];  // cyc_lds_floats(ns, P, waves, schedule) floats, sized by the launch
   const eqf::SceneDesc& sc = descs[blockIdx.x];
   // The largest scenes of a batch set the launch's length: their waves win
   // issue arbitration against co-resident smaller scenes.
@@ -786,9 +840,15 @@ __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* 
     default: break;
   }
   const int nf = sc.frame_mult > 1 ? nframes * sc.frame_mult : nframes;
-  for (int fr = 0; fr < nf; ++fr) eqf::frame_cyclic<D, B>(sc, (eqf::lf*)eqf_lds);
+  for (int fr = 0; fr < nf; ++fr) eqf::frame_cyclic<D, B, XR>(sc, (eqf::lf*)eqf_lds);
 }
 
-template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 1024>(const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<EQF_GS_D, EQF_GS_B, 512>(const eqf::SceneDesc* __restrict__, int);
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_WIDE), false, 1024>(
+    const eqf::SceneDesc* __restrict__, int);
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_WIDE), false, 512>(
+    const eqf::SceneDesc* __restrict__, int);
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_RING), true, 1024>(
+    const eqf::SceneDesc* __restrict__, int);
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_RING), true, 512>(
+    const eqf::SceneDesc* __restrict__, int);
 

@@ -3,7 +3,7 @@
 * Red-black mode: band shape x sweeps per launch (eqf_scene_set_tuning).
 * Batches (wavefront mode): waves per workgroup at most -- with fewer waves a
   strip runs fewer sweep streams -- and the scene-to-workgroup order
-  (eqf_batch_set_tuning).
+  (eqf_batch_set_tuning), and the wavefront schedule (eqf_batch_set_schedule).
 * Batches (red-black mode): band shape x sweeps per launch over all scenes
   (eqf_batch_set_band).
 
@@ -78,22 +78,23 @@ def tune_redblack(scene, frames=1, use_cache=True):
 
 BATCH_WAVES = (16, 8, 4, 2, 1)
 BATCH_ORDERS = (_abi.BATCH_LARGEST_FIRST, _abi.BATCH_GIVEN_ORDER)
+BATCH_SCHEDS = (_abi.SCHED_WIDE, _abi.SCHED_RING)
 
 
 def _batch_key(scenes):
     sig = ",".join(f"{s.params.size}:{s.params.gs_iters}:{s.params.pressure_iters}" for s in scenes)
-    return f"batch:{len(scenes)}:{hashlib.sha1(sig.encode()).hexdigest()[:16]}:{_device_key(scenes[0].ctx)}"
+    return f"batch2:{len(scenes)}:{hashlib.sha1(sig.encode()).hexdigest()[:16]}:{_device_key(scenes[0].ctx)}"
 
 
 def tune_batch(scenes, frames=2, use_cache=True):
-    """Picks (max_waves, order) for a batch of the given wavefront-mode
-    _abi.Scene objects and returns (max_waves, order, timings in ms per
-    frame).  Trials run on clones."""
+    """Picks (max_waves, order, schedule) for a batch of the given
+    wavefront-mode _abi.Scene objects and returns (max_waves, order,
+    schedule, timings in ms per frame).  Trials run on clones."""
     key = _batch_key(scenes)
     db = _load() if use_cache else {}
     if key in db:
-        w, o = db[key]["best"]
-        return w, o, db[key].get("ms", {})
+        w, o, sc = db[key]["best"]
+        return w, o, sc, db[key].get("ms", {})
     maxns = max((s.params.size - 2 + 63) // 64 for s in scenes)
     clones = [s.clone() for s in scenes]
     b = _abi.Batch(clones)
@@ -104,23 +105,25 @@ def tune_batch(scenes, frames=2, use_cache=True):
                 continue
             for o in BATCH_ORDERS:
                 b.set_tuning(w, o)
-                b.step(frames)  # builds the graph of `frames` launches
-                b.sync()
-                t0 = time.perf_counter()
-                b.step(frames)
-                b.sync()
-                timings[f"{w}:{o}"] = (time.perf_counter() - t0) * 1e3 / frames
+                for sc in BATCH_SCHEDS:
+                    b.set_schedule(sc)
+                    b.step(frames)  # builds the graph of `frames` launches
+                    b.sync()
+                    t0 = time.perf_counter()
+                    b.step(frames)
+                    b.sync()
+                    timings[f"{w}:{o}:{sc}"] = (time.perf_counter() - t0) * 1e3 / frames
     finally:
         b.close()
         for c in clones:
             c.close()
     best = min(timings, key=timings.get)
-    w, o = (int(x) for x in best.split(":"))
+    w, o, sc = (int(x) for x in best.split(":"))
     db = _load() if use_cache else {}
-    db[key] = {"best": [w, o], "ms": timings}
+    db[key] = {"best": [w, o, sc], "ms": timings}
     if use_cache:
         _save(db)
-    return w, o, timings
+    return w, o, sc, timings
 
 
 def tune_batch_band(scenes, frames=2, use_cache=True):

@@ -60,6 +60,7 @@ def test_null_and_invalid_arguments_fail_cleanly():
     assert L.eqf_step(None, 1) == _abi.EQF_EINVAL
     assert L.eqf_download(None, 0, None) == _abi.EQF_EINVAL
     assert L.eqf_batch_step(None, 1) == _abi.EQF_EINVAL
+    assert L.eqf_batch_set_schedule(None, _abi.SCHED_RING) == _abi.EQF_EINVAL
     assert "NULL" in _abi.last_error() or "null" in _abi.last_error().lower()
 
 

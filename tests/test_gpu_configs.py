@@ -94,14 +94,17 @@ def _mix_sample(per_size=16, frames=2):
     return out
 
 
-@pytest.mark.parametrize("max_waves,order", [(8, _abi.BATCH_LARGEST_FIRST), (16, _abi.BATCH_GIVEN_ORDER)])
-def test_batched_mix_sample_bit_exact(max_waves, order):
+@pytest.mark.parametrize("max_waves,order,sched", [(8, _abi.BATCH_LARGEST_FIRST, _abi.SCHED_WIDE),
+                                                   (16, _abi.BATCH_GIVEN_ORDER, _abi.SCHED_WIDE),
+                                                   (16, _abi.BATCH_LARGEST_FIRST, _abi.SCHED_RING)])
+def test_batched_mix_sample_bit_exact(max_waves, order, sched):
     sample = _mix_sample()
     assert len(sample) == 64 and sum(sc["size"] == 514 for sc in sample) == 16
-    _case(f"batched mix sample: 64 scenes, max_waves={max_waves} order={order}, 2 frames")
+    _case(f"batched mix sample: 64 scenes, max_waves={max_waves} order={order} schedule={sched}, 2 frames")
     hs = [gpu_scene(sc) for sc in sample]
     b = _abi.Batch(hs)
     b.set_tuning(max_waves, order)
+    b.set_schedule(sched)
     b.step(2)
     b.sync()
     got = [download_all(h) for h in hs]

@@ -61,11 +61,13 @@ def test_random_scenes_batched_equal_oracle():
         assert_state_equal(download_all(h), run_oracle(sc, frames=frames), f"batched random k={k + 24}")
 
 
-def test_exact_batch_of_strip_boundary_sizes_equals_oracle():
+@pytest.mark.parametrize("sched", [_abi.SCHED_AUTO, _abi.SCHED_RING])
+def test_exact_batch_of_strip_boundary_sizes_equals_oracle(sched):
     """One fused launch over scenes at and around the strip (64 rows) and
     period (16 slots) boundaries, down to size 3, with sweep counts from 3 to
     6 and frame sources on some fields only; every scene equal to the oracle
-    bit for bit."""
+    bit for bit, under the default schedule and the x0 ring schedule (sweep
+    counts below the stream count cut the streams)."""
     sizes = [1, 2, 3, 5, 15, 16, 17, 62, 63, 64, 65, 66, 126, 127, 128, 129, 130, 191, 192, 193, 255, 257]
     scs = []
     for k, N in enumerate(sizes):
@@ -76,9 +78,10 @@ def test_exact_batch_of_strip_boundary_sizes_equals_oracle():
         if k % 5 == 2:
             sc["src_d"] = None
         scs.append(sc)
-    print(f"[case] exact batch of {len(scs)} boundary sizes", flush=True)
+    print(f"[case] exact batch of {len(scs)} boundary sizes, schedule {sched}", flush=True)
     hs = [gpu_scene(sc) for sc in scs]
     b = _abi.Batch(hs)
+    b.set_schedule(sched)
     b.step(2)
     b.sync()
     b.close()

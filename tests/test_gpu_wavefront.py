@@ -268,10 +268,13 @@ def _noise_params(size):
     return p
 
 
-def test_batch_tuning_keeps_results():
+@pytest.mark.parametrize("sched", [_abi.SCHED_WIDE, _abi.SCHED_RING])
+def test_batch_tuning_keeps_results(sched):
     """Every launch shape the batch autotuner may pick (waves per workgroup,
-    workgroup order) gives the oracle's bits."""
-    _case("batch tuning")
+    workgroup order, wavefront schedule) gives the oracle's bits.  With the
+    ring schedule and 16 waves the scenes run 1 (N=40), 2 (N=70), 3 (N=130)
+    and 4 (N=200) sweep streams per strip, chaining x0 through 0 to 3 rings."""
+    _case(f"batch tuning, schedule {sched}")
     scs = [scenes.stam_scene(N, scenes.derive_seed(60, N), iters=5, frames=2, n_rects=2)
            for N in (40, 130, 200, 70)]
     want = [run_oracle(sc, frames=2) for sc in scs]
@@ -279,11 +282,12 @@ def test_batch_tuning_keeps_results():
         hs = [gpu_scene(sc) for sc in scs]
         b = _abi.Batch(hs)
         b.set_tuning(w, o)
+        b.set_schedule(sched)
         b.step(2)
         b.sync()
         b.close()
         for sc, h, st in zip(scs, hs, want):
-            assert_state_equal(download_all(h), st, f"tuning {w}:{o} N={sc['size'] - 2}")
+            assert_state_equal(download_all(h), st, f"tuning {w}:{o}:{sched} N={sc['size'] - 2}")
             h.close()
     hs = [gpu_scene(sc) for sc in scs]
     b = _abi.Batch(hs)
