@@ -133,8 +133,8 @@ EQF_HD constexpr int cyc_stream_lag(int B, int D) { return (cyc_strip_lag(B, D) 
 
 // Exact mode keeps the strips' hand-off rows in LDS: 2*ns rows of P floats
 // (top ring, raw last rows, post-boundary first rows, bottom ring) plus a
-// 16-float sink after them that absorbs the stores of lanes with nothing to
-// hand off (all such lanes of a store write the same float: no bank conflict).
+// sink row after them that absorbs the stores of lanes with nothing to hand
+// off (all such lanes of a store write the same float: no bank conflict).
 // Rows are P + 1 floats apart, so that lane 63's edge read (the strip below)
 // and the other lanes' shared read (the strip above) fall in different banks.
 EQF_HD constexpr int cyc_lds_row(int P) { return P + 1; }
@@ -156,11 +156,11 @@ EQF_HD constexpr int cyc_streams(int waves, int ns, int P, int sched) {
 // so a barrier, later.  Stream j overwrites the entry at its step tau + R,
 // which with R >= LAG + B lies in a later block than either load.
 EQF_HD constexpr int cyc_x0_ring(int B, int D) { return cyc_stream_lag(B, D) + B; }
-// LDS layout (floats): the hand-off rows and their sink, then (GS_RING, at
-// cyc_x0_base) a sink of B slot rows for x0 stores nobody reads and one ring
-// of R slot rows per pair of consecutive streams of each strip.  At most
-// 39448 floats (154 KiB, N = 241 with 16 waves) of the 160 KiB of a CU.
-EQF_HD constexpr int cyc_x0_base(int ns, int P) { return 2 * ns * cyc_lds_row(P) + 16; }
+// LDS layout (floats): the hand-off rows and their sink row, then (GS_RING,
+// at cyc_x0_base) a sink of B slot rows for x0 stores nobody reads and one
+// ring of R slot rows per pair of consecutive streams of each strip.  At
+// most 39689 floats (155 KiB, N = 241 with 16 waves) of the 160 KiB of a CU.
+EQF_HD constexpr int cyc_x0_base(int ns, int P) { return (2 * ns + 1) * cyc_lds_row(P); }
 // The one-pass phases reuse the space for the advect trigger columns (one int
 // per row: 64 per strip).
 EQF_HD constexpr int cyc_lds_floats(int ns, int P, int waves, int sched) {

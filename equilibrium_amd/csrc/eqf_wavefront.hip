@@ -60,6 +60,10 @@ __device__ __forceinline__ float from_next_lane(float v, float edge = 0.0f) {  /
   return r;
 }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// LDS pointer `bytes` past H.
+__device__ __forceinline__ lf* lds_at(lf* H, int bytes) {
+  return (lf*)((__attribute__((address_space(3))) char*)H + bytes);
+}
 
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 // Raw buffer descriptor (gfx9 family: DATA_FORMAT 32 in word 3).
@@ -295,14 +299,19 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   const int up_row = has_prev ? s : 0;                // raw row of the strip above, or the top ring
   const int dn_row = has_next ? ns + s : 2 * ns - 1;  // post row of the strip below, or the bottom ring
   const int e_row = (last_e ? dn_row : up_row) * HP;  // H row of this lane's edge value (floats)
-  // Sink for the hand-off stores of lanes with nothing to hand off: one
-  // float shared by every lane (same-address writes of a wave do not
-  // conflict; no lane reads it).
+  // Sink row for the hand-off stores of lanes with nothing to hand off: the
+  // lanes of a store all write the same float of it (same-address writes of
+  // a wave do not conflict; no lane reads it).
   const int TR = 2 * ns * HP;
   const int raw_row = (1 + s) * HP;                    // this strip's raw last row (if has_next)
   const int post_row = (ns - 1 + s) * HP;              // this strip's post row 0 (if has_prev)
   const bool raw_ok = last && valid && has_next;      // lane 63 hands its raw values down
   const bool post_ok = first && has_prev;             // lane 0 hands its final values up
+  // Byte offsets of each lane's hand-off rows (the sink row for the others),
+  // to which a block adds its uniform slot offsets.
+  const int e_b = e_row * 4;
+  const int raw_b = (raw_ok ? raw_row : TR) * 4;
+  const int post_b = (post_ok ? post_row : TR) * 4;
   const int sb = uni(s * P * 256);                        // strip base in X (bytes)
   const int cb = uni(s * P * 16);                        // strip base in the code array (bytes)
   const int fxb = uni(has_prev ? sb - P * 256 + 252 : 0);  // lane 63 of the strip above (bytes)
