@@ -69,9 +69,9 @@ int eqf_stream_l2(int maxP);
 // strips times the sweep streams per strip (up to 16 waves, at most
 // P / stream lag streams).
 int eqf_scene_waves(int ns, int P, int sched);
-// The schedule (GsSched) of a launch of n scenes whose largest has maxns
-// strips, with at most max_waves waves per workgroup.
-int eqf_pick_sched(int n, int maxns, int max_waves);
+// The schedule (GsSched) of a launch whose largest scene has maxns strips
+// and period P, with at most max_waves waves per workgroup.
+int eqf_pick_sched(int maxns, int P, int max_waves);
 // waves / lds_floats: the largest eqf_scene_waves() and cyc_lds_floats() of
 // the scenes under schedule `sched`.
 hipError_t eqf_internal_launch(const eqf::SceneDesc* d_descs, int n, int waves, int lds_floats, int nframes,

@@ -102,34 +102,41 @@ struct SceneDesc {
 };
 
 // Exact-mode wavefront schedules (compile-time per kernel, so that every
-// access offset is an immediate): prefetch depth GS_D steps, B steps between
-// workgroup barriers, and whether x0 goes from sweep stream to sweep stream
-// through LDS (cyc_x0_ring).
-//   GS_WIDE: B = 16, x0 read from HBM by every stream (fewest instructions
-//            per step: lone scenes and small launches, issue-bound)
-//   GS_RING: B = 8, x0 handed on in LDS (a shorter stream lag and fewer HBM
-//            bytes: large launches, bound by the memory system)
+// access offset is an immediate): prefetch depth GS_D steps, blocks of
+// GS_B steps (per-block set-up, one selector word), a workgroup barrier every
+// BS steps (BS divides GS_B), and whether x0 goes from sweep stream to sweep
+// stream through LDS (cyc_x0_ring).
+//   GS_WIDE: BS = 16, x0 read from HBM by every stream (the fewest barriers:
+//            lone scenes and small launches, issue-bound)
+//   GS_RING: BS = 8, x0 handed on in LDS (a shorter stream lag that lets the
+//            rings fit, and fewer HBM bytes: large launches, bound by the
+//            memory system)
 enum GsSched { GS_WIDE = 0, GS_RING = 1 };
 constexpr int GS_D = 8;
-EQF_HD constexpr int gs_block(int sched) { return sched == GS_RING ? 8 : 16; }
+constexpr int GS_B = 16;
+EQF_HD constexpr int gs_sync(int sched) { return sched == GS_RING ? 8 : 16; }
 
 EQF_HD constexpr int cyc_period(int N) { return (N + 15) & ~15; }
 
-// Wavefront lags for blocks of B steps (B divides 16) and prefetch depth
-// D <= B, both multiples of B.  A value handed from one wave to another must be read
-// at least B steps after it was stored (a block barrier lies in between
-// whatever the alignment), and reads are issued D steps before use:
-//  * strip s+1 trails strip s by L >= 64 + B + D steps: its lane 0 reads strip
+EQF_HD constexpr int round_up(int v, int q) { return (v + q - 1) / q * q; }
+// Wavefront lags for blocks of B steps, barriers every BS steps and prefetch
+// depth D <= BS, multiples of B so that every stream's blocks start on
+// a multiple of B global steps.  A value handed from one wave to another
+// must be read at least BS steps after it was stored (a barrier lies in
+// between whatever the alignment), and reads are issued D steps before use:
+//  * strip s+1 trails strip s by L >= 64 + BS + D steps: its lane 0 reads strip
 //    s's raw last-row value of column c, stored at strip-local step c + 62,
 //    at step c - 1 - D of its own;
-//  * sweep stream j+1 of a strip trails stream j by LAG >= L - 62 + B + D: its
+//  * sweep stream j+1 of a strip trails stream j by LAG >= L - 62 + BS + D: its
 //    last lane reads the strip below's post-boundary row-0 value of column c,
 //    which stream j of that strip stores at its step c (AdjustRow /
 //    AdjustColumn store a cell's final value one step after computing it),
 //    at step c + 62 - D of its own (the AdjustColumn fix-up is stored a step
 //    earlier).
-EQF_HD constexpr int cyc_strip_lag(int B, int D) { return (64 + B + D + B - 1) / B * B; }
-EQF_HD constexpr int cyc_stream_lag(int B, int D) { return (cyc_strip_lag(B, D) - 62 + B + D + B - 1) / B * B; }
+EQF_HD constexpr int cyc_strip_lag(int B, int BS, int D) { return round_up(64 + BS + D, B); }
+EQF_HD constexpr int cyc_stream_lag(int B, int BS, int D) {
+  return round_up(cyc_strip_lag(B, BS, D) - 62 + BS + D, B);
+}
 
 // Exact mode keeps the strips' hand-off rows in LDS: 2*ns rows of P floats
 // (top ring, raw last rows, post-boundary first rows, bottom ring) plus a
@@ -138,8 +145,8 @@ EQF_HD constexpr int cyc_stream_lag(int B, int D) { return (cyc_strip_lag(B, D) 
 // Rows are P + 1 floats apart, so that lane 63's edge read (the strip below)
 // and the other lanes' shared read (the strip above) fall in different banks.
 EQF_HD constexpr int cyc_lds_row(int P) { return P + 1; }
-EQF_HD constexpr int gs_strip_lag(int sched) { return cyc_strip_lag(gs_block(sched), GS_D); }
-EQF_HD constexpr int gs_stream_lag(int sched) { return cyc_stream_lag(gs_block(sched), GS_D); }
+EQF_HD constexpr int gs_strip_lag(int sched) { return cyc_strip_lag(GS_B, gs_sync(sched), GS_D); }
+EQF_HD constexpr int gs_stream_lag(int sched) { return cyc_stream_lag(GS_B, gs_sync(sched), GS_D); }
 // Sweep streams per strip of a scene run by `waves` waves (lin_solves with
 // fewer sweeps than that use fewer).
 EQF_HD constexpr int cyc_streams(int waves, int ns, int P, int sched) {
@@ -148,36 +155,44 @@ EQF_HD constexpr int cyc_streams(int waves, int ns, int P, int sched) {
   m = m < mp ? m : mp;
   return m > 1 ? m : 1;
 }
-// x0 (a lin_solve's right-hand side, the same in every sweep) goes from each
-// sweep stream of a strip to the next through a ring of slot rows in LDS
+// x0 (a lin_solve's right-hand side, the same in every sweep) goes from a
+// sweep stream of a strip to the next through a ring of R slot rows in LDS
 // instead of being read from HBM again: stream j stores the 64 values of its
 // step tau at entry tau mod R, and stream j+1 loads them D steps before its
-// own step tau (its first D entries in its first block), LAG - D >= B steps,
-// so a barrier, later.  Stream j overwrites the entry at its step tau + R,
-// which with R >= LAG + B lies in a later block than either load.
-EQF_HD constexpr int cyc_x0_ring(int B, int D) { return cyc_stream_lag(B, D) + B; }
+// own step tau (its first D entries before its first barrier), LAG - D >= BS
+// steps, so a barrier, later.  Stream j overwrites the entry at its step
+// tau + R, which with R >= LAG + BS lies after a later barrier than either
+// load.  R is a multiple of BS: the BS steps between two barriers never wrap.
+EQF_HD constexpr int cyc_x0_ring(int B, int BS, int D) { return round_up(cyc_stream_lag(B, BS, D) + BS, BS); }
 // LDS layout (floats): the hand-off rows and their sink row, then (GS_RING,
-// at cyc_x0_base) a sink of B slot rows for x0 stores nobody reads and one
-// ring of R slot rows per pair of consecutive streams of each strip.  At
-// most 39689 floats (155 KiB, N = 241 with 16 waves) of the 160 KiB of a CU.
+// at cyc_x0_base) a sink of GS_B slot rows for x0 stores nobody reads and
+// rings of R slot rows for the pairs of consecutive streams of the strips,
+// strip by strip, as many as fit in a CU's 160 KiB (cyc_x0_rings); streams
+// past them read x0 from HBM.
+constexpr int LDS_FLOATS_MAX = 160 * 1024 / 4;
 EQF_HD constexpr int cyc_x0_base(int ns, int P) { return (2 * ns + 1) * cyc_lds_row(P); }
+EQF_HD constexpr int cyc_x0_rings(int ns, int P, int m) {
+  const int R = cyc_x0_ring(GS_B, gs_sync(GS_RING), GS_D);
+  const int fit = (LDS_FLOATS_MAX - cyc_x0_base(ns, P) - GS_B * 64) / (R * 64);
+  const int want = ns * (m - 1);
+  return want < fit ? want : (fit > 0 ? fit : 0);
+}
 // The one-pass phases reuse the space for the advect trigger columns (one int
 // per row: 64 per strip).
 EQF_HD constexpr int cyc_lds_floats(int ns, int P, int waves, int sched) {
   int h = cyc_x0_base(ns, P);
-  if (sched == GS_RING) {
-    const int B = gs_block(GS_RING);
-    h += B * 64 + ns * (cyc_streams(waves, ns, P, sched) - 1) * cyc_x0_ring(B, GS_D) * 64;
-  }
+  if (sched == GS_RING)
+    h += GS_B * 64 + cyc_x0_rings(ns, P, cyc_streams(waves, ns, P, sched)) *
+                         cyc_x0_ring(GS_B, gs_sync(GS_RING), GS_D) * 64;
   return h > 64 * ns ? h : 64 * ns;
 }
 // Every exact-mode launch (N up to 1024, 1..16 waves, either schedule) fits
-// the 160 KiB of LDS of a CU.
+// the 160 KiB of LDS of a CU (N = P - 15 .. P share the strip count and period).
 constexpr bool cyc_lds_fits() {
   for (int sched = GS_WIDE; sched <= GS_RING; ++sched)
     for (int w = 1; w <= 16; ++w)
-      for (int n = 1; n <= 1024; ++n)
-        if (cyc_lds_floats((n + 63) / 64, cyc_period(n), w, sched) > 160 * 1024 / 4) return false;
+      for (int P = 16; P <= 1024; P += 16)
+        if (cyc_lds_floats((P + 63) / 64, P, w, sched) > LDS_FLOATS_MAX) return false;
   return true;
 }
 static_assert(cyc_lds_fits(), "an exact-mode launch would need more LDS than a CU has");

@@ -19,7 +19,7 @@
 
 using namespace eqf;
 
-template <int D, int B, bool XR, int MAXT>
+template <int D, int B, int BS, bool XR, int MAXT>
 __global__ void eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes);
 
 
@@ -571,12 +571,11 @@ extern "C" int eqf_step(eqf_scene* s, int64_t nframes) {
     rc = rb_step(s, nframes, err);
     return rc ? fail(rc, err) : EQF_OK;
   }
-  // a lone scene is instruction-issue-bound: the schedule with the fewest
-  // instructions per step
-  const int W = eqf_scene_waves(s->ns, s->P, GS_WIDE);
+  const int sched = eqf_pick_sched(s->ns, s->P, 16);
+  const int W = eqf_scene_waves(s->ns, s->P, sched);
   while (nframes > 0) {
     const int nf = (int)std::min<int64_t>(nframes, 1 << 20);
-    HIPCHK(eqf_internal_launch(s->d_desc, 1, W, cyc_lds_floats(s->ns, s->P, W, GS_WIDE), nf, GS_WIDE, s->stream));
+    HIPCHK(eqf_internal_launch(s->d_desc, 1, W, cyc_lds_floats(s->ns, s->P, W, sched), nf, sched, s->stream));
     nframes -= nf;
   }
   return EQF_OK;
@@ -630,14 +629,16 @@ static int batch_layout(eqf_batch* b) {
   }
   const int n = (int)b->scenes.size();
   std::vector<SceneDesc> descs(n);
-  int maxns = 1, maxw = 1;
+  int maxns = 1, maxw = 1, bigP = 16;  // bigP: the period of the largest scene (most strips)
   for (int i = 0; i < n; ++i) {
     descs[i] = b->scenes[i]->desc;
-    maxns = std::max(maxns, b->scenes[i]->ns);
+    const eqf_scene* s = b->scenes[i];
+    if (s->ns > maxns || (s->ns == maxns && s->P > bigP)) bigP = s->P;
+    maxns = std::max(maxns, s->ns);
   }
   b->sched = b->sched_req == EQF_SCHED_RING   ? GS_RING
              : b->sched_req == EQF_SCHED_WIDE ? GS_WIDE
-                                              : eqf_pick_sched(n, maxns, b->max_waves);
+                                              : eqf_pick_sched(maxns, bigP, b->max_waves);
   for (int i = 0; i < n; ++i) maxw = std::max(maxw, eqf_scene_waves(b->scenes[i]->ns, b->scenes[i]->P, b->sched));
   if (b->max_waves < maxns) return fail(EQF_EINVAL, "max_waves below the largest scene's strip count");
   int maxP = 0;
@@ -770,35 +771,40 @@ extern "C" int eqf_batch_set_tuning(eqf_batch* b, int max_waves, int order) {
 hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int W, int lds_floats, int nframes, int sched,
                                hipStream_t st) {
   const size_t lds = sizeof(float) * (size_t)lds_floats;
-  constexpr int BW = gs_block(GS_WIDE), BR = gs_block(GS_RING);
+  constexpr int SW = gs_sync(GS_WIDE), SR = gs_sync(GS_RING);
   if (sched == GS_RING) {
     if (W <= 8)
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BR, true, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SR, true, 512>), dim3(n), dim3(64 * W), lds, st, d_descs,
+                         nframes);
     else
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BR, true, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SR, true, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
+                         nframes);
   } else {
     if (W <= 8)
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BW, false, 512>), dim3(n), dim3(64 * W), lds, st, d_descs, nframes);
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SW, false, 512>), dim3(n), dim3(64 * W), lds, st, d_descs,
+                         nframes);
     else
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, BW, false, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
+      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SW, false, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
                          nframes);
   }
   return hipGetLastError();
 }
 
-// Wavefront schedule of a launch of n scenes whose largest has maxns strips,
-// with at most max_waves waves per workgroup.  GS_RING cuts the HBM bytes of
-// scenes with several sweep streams per strip but runs ~20% more
-// instructions per step: it pays where the memory system, not instruction
-// issue, sets the frame time, as with many scenes of 5 to 8 strips (N
-// 257..512) that get two or more streams each; a largest scene with one
-// stream gains nothing from the ring.  Measured (ms per frame, 20 sweeps,
-// wide / ring): 16 waves: 121 scenes N=512 11.03 / 10.50, the 512-scene mix
-// 12.24 / 11.93, 32 scenes N=512 7.80 / 8.19, one N=512 7.43 / 7.80, 130
-// N=256 2.52 / 2.74, 391 scenes N <= 256 3.10 / 3.07-3.11; 8 waves (one
-// stream per strip at N=512): 121 scenes N=512 12.12 / 14.17.
-int eqf_pick_sched(int n, int maxns, int max_waves) {
-  return (n >= 64 && maxns >= 5 && max_waves / maxns >= 2) ? GS_RING : GS_WIDE;
+// Wavefront schedule of a launch whose largest scene (the longest frame
+// chain) has maxns strips and period P, with at most max_waves waves per
+// workgroup.  GS_RING hands x0 between sweep streams in LDS (fewer
+// vector-memory instructions and HBM bytes) but needs a barrier every 8
+// steps: it pays when that scene runs two or more streams per strip and at
+// least 8 waves; with one stream, or few waves per CU, the barriers cost
+// more than the ring saves.  Measured (ms per frame, 20 sweeps, 16 waves,
+// wide / ring, one box): one N=512 7.42 / 6.96, 32 N=512 7.78 / 7.37, 121
+// N=512 11.02 / 10.00, the 512-scene mix 12.25 / 11.52, one N=256 2.12 /
+// 2.02, 130 N=256 2.88 / 2.40, 391 scenes N <= 256 2.95-3.18 / 2.95-2.98, one
+// N=128 (2 strips x 2 streams) 1.16 / 1.24, one N=64 (one stream) 0.94 /
+// 0.96; 121 N=512 with 8 waves (one stream per strip) 12.06 / 12.36.
+int eqf_pick_sched(int maxns, int P, int max_waves) {
+  const int m = cyc_streams(max_waves, maxns, P, GS_RING);
+  return (m >= 2 && maxns * m >= 8) ? GS_RING : GS_WIDE;
 }
 
 // Non-temporal streaming (SceneDesc::stream_l2) for the scenes of a launch

@@ -243,8 +243,12 @@ static std::string lane_launch(eqf_server* sv, int li) {
     std::stable_sort(wf.begin(), wf.end(), [](const Req* a, const Req* b) { return a->cells > b->cells; });
     int maxns = 1, lds = 0, waves = 1;
     double chain_max = 0;
-    for (Req* r : wf) maxns = std::max(maxns, r->scene->ns);
-    const int sched = eqf_pick_sched((int)wf.size(), maxns, 16);
+    int bigP = 16;  // the period of the largest scene (most strips)
+    for (Req* r : wf) {
+      if (r->scene->ns > maxns || (r->scene->ns == maxns && r->scene->P > bigP)) bigP = r->scene->P;
+      maxns = std::max(maxns, r->scene->ns);
+    }
+    const int sched = eqf_pick_sched(maxns, bigP, 16);
     for (Req* r : wf) {
       waves = std::max(waves, eqf_scene_waves(r->scene->ns, r->scene->P, sched));
       chain_max = std::max(chain_max, frame_chain(r->scene, sched));

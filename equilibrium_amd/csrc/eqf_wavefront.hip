@@ -184,20 +184,22 @@ __device__ __forceinline__ uint32_t step_bits(int lo, int hi, int B) {
 // sweep reads such a ring value as `xself`, the previous sweep's value of the
 // cell being updated, and the ring itself is written once after the last
 // sweep by the list pass.
-template <int MODE, int D, int B, bool XR>
+template <int MODE, int D, int B, int BS, bool XR>
 __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict__ H, gf* __restrict__ X,
                                              gcf* __restrict__ X0, float a, float cr, int iters) {
   static_assert((B == 16 || B == 8) && D <= B, "blocks of 8 or 16 steps (within one selector word), at least "
                                               "as deep as the prefetch");
+  static_assert(BS >= D && B % BS == 0, "barriers every BS steps: BS divides the block, at least the prefetch");
   // the prefetch registers form a ring indexed by the step within the block
   // modulo D, which carries over from block to block only if D divides B
   static_assert(B % D == 0, "prefetch depth must divide the block length");
-  constexpr int L = cyc_strip_lag(B, D);    // strip lag, a multiple of B
-  constexpr int LAG = cyc_stream_lag(B, D);  // sweep lag between streams of a strip
-  // within a strip, stream j+1 loads a slot stream j stored B + D + 2 steps
+  constexpr int L = cyc_strip_lag(B, BS, D);    // strip lag, a multiple of B
+  constexpr int LAG = cyc_stream_lag(B, BS, D);  // sweep lag between streams of a strip
+  // within a strip, stream j+1 loads a slot stream j stored BS + D + 2 steps
   // earlier at least (one step of delayed store, one of prefetch offset)
-  static_assert(LAG >= B + D + 2, "stream lag too short for the prefetch");
-  static_assert(L >= 64 + B + D && LAG >= L - 62 + B + D, "hand-off lags too short");
+  static_assert(LAG >= BS + D + 2, "stream lag too short for the prefetch");
+  static_assert(L >= 64 + BS + D && LAG >= L - 62 + BS + D && L % B == 0 && LAG % B == 0,
+                "hand-off lags too short");
   const int N = sc.N, S = sc.S, ns = sc.ns, P = sc.P;
   const int lane = threadIdx.x & 63;
   const int wave = uni(threadIdx.x >> 6);
@@ -227,15 +229,18 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   // lines the other streams are about to re-read (batched mix: 12.97 ->
   // 12.32 ms per frame); only where SceneDesc::stream_l2 says so.
   const bool nt_stream = m > 1 && jst == m - 1 && sc.stream_l2 != 0;
-  // x0 hand-off ring (cyc_x0_ring): stream jst > 0 loads x0 from ring
-  // s*(m-1) + jst-1 in LDS instead of HBM; every stream but the last stores
-  // the x0 of each step into its own ring s*(m-1) + jst (the others into a
-  // sink that nobody reads).  Ring entries are slot rows indexed by the
-  // stream-local step modulo R, so writer and reader agree on them.
-  constexpr int R = cyc_x0_ring(B, D);
-  static_assert(!XR || (R % B == 0 && R >= LAG + B && LAG - D >= B), "x0 ring too short or read too early");
-  const bool rin = XR && has_strip && jst > 0;
-  const bool rout = XR && has_strip && jst < m - 1;
+  // x0 hand-off rings (cyc_x0_ring): stream jst > 0 loads x0 from ring
+  // s*(m-1) + jst-1 in LDS instead of HBM, and every stream but the last
+  // stores the x0 of each step into its own ring s*(m-1) + jst (the others
+  // into a sink that nobody reads), for the rings that fit (cyc_x0_rings;
+  // the launch sized LDS for at least as many).  Ring entries are slot rows
+  // indexed by the stream-local step modulo R, so writer and reader agree.
+  constexpr int R = cyc_x0_ring(B, BS, D);
+  static_assert(!XR || (R % BS == 0 && R >= LAG + BS && LAG - D >= BS && R >= B),
+                "x0 ring too short or read too early");
+  const int nrings = XR ? cyc_x0_rings(ns, P, m) : 0;
+  const bool rin = XR && has_strip && jst > 0 && s * (m - 1) + jst - 1 < nrings;
+  const bool rout = XR && has_strip && jst < m - 1 && s * (m - 1) + jst < nrings;
   const int xsink = cyc_x0_base(ns, P);
   const int xring = xsink + B * 64 + (s * (m - 1) + jst) * R * 64;  // this stream's ring (if rout)
   const int xo_ring = rout ? xring : -1;           // < 0: stores go to the sink
@@ -342,7 +347,9 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   // (columns past N, never read as cell values) and store unconditionally.
   int blk = 0;
 #pragma nounroll
-  for (; blk < head; ++blk) __syncthreads();
+  for (; blk < head; ++blk)
+#pragma unroll
+    for (int k = 0; k < B / BS; ++k) __syncthreads();
   if (has_strip) {
 #pragma unroll
     for (int i = 0; i < D; ++i) {  // prefetch the first D steps (slot i)
@@ -360,12 +367,23 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
       if (tau >= 64 && tau + B <= itP) {
         // interior blocks, by where x0 comes from (XS: 0 HBM, 1 the ring)
         // and whether it is handed on (XW)
-        if (nt_stream) {
+        if (nt_stream && (rin || !XR)) {
 #pragma nounroll
           do {
             {
               constexpr bool EDGE = false;
               constexpr int POL = CP_NT, XS = XR ? 1 : 0;
+              constexpr bool XW = false;
+#include "eqf_gs_block.inc"
+            }
+            ++blk;
+          } while (tau + B <= itP);
+        } else if (XR && nt_stream) {  // the last stream of a strip past the rings that fit
+#pragma nounroll
+          do {
+            {
+              constexpr bool EDGE = false;
+              constexpr int POL = CP_NT, XS = 0;
               constexpr bool XW = false;
 #include "eqf_gs_block.inc"
             }
@@ -426,7 +444,8 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
         }
         ++blk;
       } else {
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < B / BS; ++k) __syncthreads();
         ++blk;
       }
     }
@@ -776,7 +795,7 @@ enum PhaseKind { PK_GS, PK_DIV, PK_GRAD, PK_ADV, PK_COPY };
 // divergence of the first projection, U0/V0 as those of the second.  The
 // phases run from a loop so that each phase kernel body exists once in the
 // code object.
-template <int D, int B, bool XR>
+template <int D, int B, int BS, bool XR>
 __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
   gf* f = (gf*)sc.fld;
   const int n = (int)sc.fstride;
@@ -812,9 +831,9 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
     }
     if (kind == PK_GS) {
       if (iters > 0) {
-        if (mode == M_ROW) gs_wavefront<M_ROW, D, B, XR>(sc, H, x, y, a, cr, iters);
-        else if (mode == M_COL) gs_wavefront<M_COL, D, B, XR>(sc, H, x, y, a, cr, iters);
-        else gs_wavefront<M_PASSIVE, D, B, XR>(sc, H, x, y, a, cr, iters);
+        if (mode == M_ROW) gs_wavefront<M_ROW, D, B, BS, XR>(sc, H, x, y, a, cr, iters);
+        else if (mode == M_COL) gs_wavefront<M_COL, D, B, BS, XR>(sc, H, x, y, a, cr, iters);
+        else gs_wavefront<M_PASSIVE, D, B, BS, XR>(sc, H, x, y, a, cr, iters);
       }
     } else if (kind == PK_DIV) {
       if (big) div_phase<true>(sc, x, y, z, w);
@@ -834,9 +853,9 @@ __device__ __forceinline__ void frame_cyclic(const SceneDesc& sc, lf* H) {
 
 }  // namespace eqf
 
-// grid = number of scenes, block = 64 * waves; scene b is descs[b].  D, B
-// and XR are the launch's schedule (GsSched).
-template <int D, int B, bool XR, int MAXT>
+// grid = number of scenes, block = 64 * waves; scene b is descs[b].  D, B,
+// BS and XR are the launch's schedule (GsSched).
+template <int D, int B, int BS, bool XR, int MAXT>
 __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* __restrict__ descs, int nframes) {
   extern __shared__ float eqf_lds[];  // cyc_lds_floats(ns, P, waves, schedule) floats, sized by the launch
   const eqf::SceneDesc& sc = descs[blockIdx.x];
@@ -849,15 +868,15 @@ __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* 
     default: break;
   }
   const int nf = sc.frame_mult > 1 ? nframes * sc.frame_mult : nframes;
-  for (int fr = 0; fr < nf; ++fr) eqf::frame_cyclic<D, B, XR>(sc, (eqf::lf*)eqf_lds);
+  for (int fr = 0; fr < nf; ++fr) eqf::frame_cyclic<D, B, BS, XR>(sc, (eqf::lf*)eqf_lds);
 }
 
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_WIDE), false, 1024>(
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_WIDE), false, 1024>(
     const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_WIDE), false, 512>(
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_WIDE), false, 512>(
     const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_RING), true, 1024>(
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_RING), true, 1024>(
     const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::gs_block(eqf::GS_RING), true, 512>(
+template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_RING), true, 512>(
     const eqf::SceneDesc* __restrict__, int);
 

@@ -180,13 +180,12 @@ int eqf_batch_set_tuning(eqf_batch *b, int max_waves, int order);
 /* Wavefront schedule of a wavefront-mode batch (no counterpart in the
  * reference; results do not depend on it): EQF_SCHED_WIDE, a workgroup
  * barrier every 16 steps, every sweep stream reading a lin_solve's
- * right-hand side from device memory (fewest instructions per step);
- * EQF_SCHED_RING, a barrier every 8 steps, the right-hand side handed from
- * sweep stream to sweep stream in LDS (fewer device-memory bytes, for
- * launches bound by the memory system); EQF_SCHED_AUTO (the default): RING
- * for 64 or more scenes whose largest has N 257..512 and two or more sweep
- * streams per strip (max_waves at least twice its strip count), else WIDE.  Single
- * scenes (eqf_step) run WIDE; server ticks AUTO.  Red-black batches:
+ * right-hand side from device memory; EQF_SCHED_RING, a barrier every 8
+ * steps, the right-hand side handed from sweep stream to sweep stream in
+ * LDS (fewer device-memory reads); EQF_SCHED_AUTO (the default): RING when
+ * the batch's largest scene runs two or more sweep streams per strip and at
+ * least 8 waves (N 129..512 with max_waves 16), else WIDE.  Single scenes
+ * (eqf_step) and server ticks use the AUTO rule.  Red-black batches:
  * EQF_EINVAL. */
 enum { EQF_SCHED_AUTO = 0, EQF_SCHED_WIDE = 1, EQF_SCHED_RING = 2 };
 int eqf_batch_set_schedule(eqf_batch *b, int sched);
