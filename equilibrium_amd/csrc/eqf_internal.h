@@ -64,7 +64,7 @@ int eqf_internal_prepare(eqf_scene* s);  // boundary lists + device descriptor
 // Issue priority of a scene among the scenes of one launch: 3 for the largest
 // (strip count), then 2, 1, 0 for each halving.
 int eqf_batch_prio(int ns, int maxns);
-int eqf_stream_l2(int maxP);
+int eqf_stream_l2(int maxP, int n);
 // Waves a wavefront-mode scene uses under schedule `sched` (GsSched): its
 // strips times the sweep streams per strip (up to 16 waves, at most
 // P / stream lag streams).

@@ -96,9 +96,11 @@ struct SceneDesc {
   int prio;              // wave issue priority 0..3 in a batch launch (largest scenes highest)
   int frame_mult;        // frames per launch frame (> 1: a small scene of a server lane takes
                          // several frames in one tick of the lane's largest); 0 or 1: one
-  int stream_l2;         // exact mode: 1 = the one-pass phases and the last sweep stream of a
-                         // strip load and store non-temporally (fields too large to find in
-                         // L2 again, or a launch shared with such scenes)
+  int stream_l2;         // exact mode: >= 1: the one-pass phases and the last sweep stream of
+                         // a strip load and store non-temporally (fields too large to find in
+                         // L2 again, or a launch shared with such scenes); 2: so do the x0
+                         // loads of a strip's first stream under the ring schedule (launches
+                         // of many large scenes)
 };
 
 // Exact-mode wavefront schedules (compile-time per kernel, so that every

@@ -145,7 +145,7 @@ static void fill_desc(eqf_scene* s) {
   d.sfm1 = d.sf - 1.0f;
   d.has_src = s->d_src ? s->src_mask : 0;
   d.prio = 0;
-  d.stream_l2 = eqf_stream_l2(s->P);
+  d.stream_l2 = eqf_stream_l2(s->P, 1);
 }
 
 static int sync_desc(eqf_scene* s) {
@@ -645,7 +645,7 @@ static int batch_layout(eqf_batch* b) {
   for (int i = 0; i < n; ++i) maxP = std::max(maxP, b->scenes[i]->P);
   for (int i = 0; i < n; ++i) {
     descs[i].prio = eqf_batch_prio(b->scenes[i]->ns, maxns);
-    descs[i].stream_l2 = eqf_stream_l2(maxP);
+    descs[i].stream_l2 = eqf_stream_l2(maxP, n);
   }
   b->W = std::min(maxw, b->max_waves);
   hipError_t e =
@@ -807,13 +807,17 @@ int eqf_pick_sched(int maxns, int P, int max_waves) {
   return (m >= 2 && maxns * m >= 8) ? GS_RING : GS_WIDE;
 }
 
-// Non-temporal streaming (SceneDesc::stream_l2) for the scenes of a launch
-// whose largest scene has period P: fields of P > 256 (over 256 KB each) are
-// not found in L2 again by the next phase, and smaller scenes sharing their
-// launch leave L2 to the big scenes' sweep re-reads (batched mix 12.4 ->
-// 12.2 ms per frame); a launch of small scenes only keeps L2 for itself
-// (391 scenes N <= 256: 3.27 -> 3.00 ms).
-int eqf_stream_l2(int maxP) { return maxP > 256 ? 1 : 0; }
+// Non-temporal streaming (SceneDesc::stream_l2) for the n scenes of a launch
+// whose largest scene has period maxP: fields of P > 256 (over 256 KB each)
+// are not found in L2 again by the next phase, and smaller scenes sharing
+// their launch leave L2 to the big scenes' sweep re-reads (batched mix 12.4
+// -> 12.2 ms per frame, wide schedule); a launch of small scenes only keeps
+// L2 for itself (391 scenes N <= 256: 3.27 -> 3.00 ms).  Level 2 also
+// streams the x0 loads of ring writers (their lines' last uses): with many
+// large scenes sharing L2 it pays (the 512-scene mix 11.49 -> 10.98 ms, 121
+// N=512 9.99 -> 9.85), with few it costs (one N=512 6.93 -> 7.11, 32 N=512
+// 7.38 -> 7.81).
+int eqf_stream_l2(int maxP, int n) { return maxP > 256 ? (n >= 64 ? 2 : 1) : 0; }
 
 int eqf_batch_prio(int ns, int maxns) {
   int p = 3;

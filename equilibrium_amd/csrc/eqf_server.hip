@@ -270,7 +270,7 @@ static std::string lane_launch(eqf_server* sv, int li) {
         L.descs[i] = wf[i]->scene->desc;
         L.descs[i].prio = eqf_batch_prio(wf[i]->scene->ns, maxns);
         L.descs[i].frame_mult = wf[i]->tick_frames;
-        L.descs[i].stream_l2 = eqf_stream_l2(maxP);
+        L.descs[i].stream_l2 = eqf_stream_l2(maxP, (int)wf.size());
       }
       L.last_uids.clear();
       if ((int)wf.size() > L.desc_cap) {

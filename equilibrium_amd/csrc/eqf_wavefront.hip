@@ -374,6 +374,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
               constexpr bool EDGE = false;
               constexpr int POL = CP_NT, XS = XR ? 1 : 0;
               constexpr bool XW = false;
+              constexpr int X0POL = POL;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -385,6 +386,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
               constexpr bool EDGE = false;
               constexpr int POL = CP_NT, XS = 0;
               constexpr bool XW = false;
+              constexpr int X0POL = POL;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -396,6 +398,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
               constexpr bool EDGE = false;
               constexpr int POL = 0, XS = 1;
               constexpr bool XW = true;
+              constexpr int X0POL = POL;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -407,6 +410,21 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
               constexpr bool EDGE = false;
               constexpr int POL = 0, XS = 1;
               constexpr bool XW = false;
+              constexpr int X0POL = POL;
+#include "eqf_gs_block.inc"
+            }
+            ++blk;
+          } while (tau + B <= itP);
+        } else if (XR && rout && sc.stream_l2 > 1) {
+          // the first stream of a strip with a ring, in a launch of many
+          // large scenes: its x0 loads are the lines' last uses
+#pragma nounroll
+          do {
+            {
+              constexpr bool EDGE = false;
+              constexpr int POL = 0, XS = 0;
+              constexpr bool XW = true;
+              constexpr int X0POL = CP_NT;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -418,6 +436,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
               constexpr bool EDGE = false;
               constexpr int POL = 0, XS = 0;
               constexpr bool XW = true;
+              constexpr int X0POL = POL;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -429,6 +448,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
               constexpr bool EDGE = false;
               constexpr int POL = 0, XS = 0;
               constexpr bool XW = false;
+              constexpr int X0POL = POL;
 #include "eqf_gs_block.inc"
             }
             ++blk;
@@ -440,6 +460,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
           constexpr bool EDGE = true;
           constexpr int POL = 0, XS = XR ? 2 : 0;
           constexpr bool XW = XR;
+          constexpr int X0POL = POL;
 #include "eqf_gs_block.inc"
         }
         ++blk;
