@@ -793,18 +793,23 @@ hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int W, int lds_f
 // Wavefront schedule of a launch whose largest scene (the longest frame
 // chain) has maxns strips and period P, with at most max_waves waves per
 // workgroup.  GS_RING hands x0 between sweep streams in LDS (fewer
-// vector-memory instructions and HBM bytes) but needs a barrier every 8
-// steps: it pays when that scene runs two or more streams per strip and at
-// least 8 waves; with one stream, or few waves per CU, the barriers cost
-// more than the ring saves.  Measured (ms per frame, 20 sweeps, 16 waves,
-// wide / ring, one box): one N=512 7.42 / 6.96, 32 N=512 7.78 / 7.37, 121
-// N=512 11.02 / 10.00, the 512-scene mix 12.25 / 11.52, one N=256 2.12 /
-// 2.02, 130 N=256 2.88 / 2.40, 391 scenes N <= 256 2.95-3.18 / 2.95-2.98, one
-// N=128 (2 strips x 2 streams) 1.16 / 1.24, one N=64 (one stream) 0.94 /
-// 0.96; 121 N=512 with 8 waves (one stream per strip) 12.06 / 12.36.
+// vector-memory instructions and HBM bytes) and, with its shorter stream
+// lag, can run more streams per strip (P / 48 instead of P / 64), but needs
+// a barrier every 8 steps: it pays when it gives that scene more streams,
+// or two or more streams per strip and at least 8 waves; with the same one
+// or two streams and few waves per CU the barriers cost more than the ring
+// saves.  Measured (ms per frame, 20 sweeps, 16 waves at most, wide / ring,
+// one box): single scenes N=100 (1 / 2 streams) 1.72 / 1.11, N=128 (2 / 2
+// streams, 4 waves) 1.14 / 1.22, N=150 1.57 / 1.35, N=192 1.52 / 1.34, N=256
+// 2.11 / 2.02, N=320 3.20 / 3.00, N=448 6.02 / 5.65, N=512 7.37 / 6.98, N=600
+// (one stream) 12.97 / 13.38; batches: 32 N=512 7.78 / 7.37, 121 N=512
+// 11.02 / 10.00, the 512-scene mix 12.25 / 11.52, 130 N=256 2.88 / 2.40, 391
+// scenes N <= 256 2.95-3.18 / 2.95-2.98, 121 N=512 with 8 waves (one stream
+// per strip) 12.06 / 12.36.
 int eqf_pick_sched(int maxns, int P, int max_waves) {
   const int m = cyc_streams(max_waves, maxns, P, GS_RING);
-  return (m >= 2 && maxns * m >= 8) ? GS_RING : GS_WIDE;
+  const int mw = cyc_streams(max_waves, maxns, P, GS_WIDE);
+  return (m > mw || (m >= 2 && maxns * m >= 8)) ? GS_RING : GS_WIDE;
 }
 
 // Non-temporal streaming (SceneDesc::stream_l2) for the n scenes of a launch

@@ -182,9 +182,11 @@ int eqf_batch_set_tuning(eqf_batch *b, int max_waves, int order);
  * barrier every 16 steps, every sweep stream reading a lin_solve's
  * right-hand side from device memory; EQF_SCHED_RING, a barrier every 8
  * steps, the right-hand side handed from sweep stream to sweep stream in
- * LDS (fewer device-memory reads); EQF_SCHED_AUTO (the default): RING when
- * the batch's largest scene runs two or more sweep streams per strip and at
- * least 8 waves (N 129..512 with max_waves 16), else WIDE.  Single scenes
+ * LDS (fewer device-memory reads) and, with a shorter lag between the
+ * streams, up to P/48 streams per strip instead of P/64; EQF_SCHED_AUTO (the
+ * default): RING when it gives the batch's largest scene more sweep streams
+ * per strip, or two or more streams and at least 8 waves (N 81..112 and
+ * 129..512 with max_waves 16), else WIDE.  Single scenes
  * (eqf_step) and server ticks use the AUTO rule.  Red-black batches:
  * EQF_EINVAL. */
 enum { EQF_SCHED_AUTO = 0, EQF_SCHED_WIDE = 1, EQF_SCHED_RING = 2 };
