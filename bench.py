@@ -99,7 +99,7 @@ def device_sync():
         pass
 
 
-SCHED_NAMES = {0: "auto", 1: "wide", 2: "ring"}  # eqf_batch_set_schedule
+SCHED_NAMES = {0: "auto", 1: "wide", 2: "ring", 3: "fine"}  # eqf_batch_set_schedule
 
 
 def make_scenes(scs):
@@ -608,7 +608,7 @@ def main():
     ap.add_argument("--rb-check-frames", type=int, default=20)
     ap.add_argument("--batch-tuning", type=str, default="",
                     help="max_waves:order[:schedule] for the batch instead of autotuning (e.g. 16:0:2; "
-                         "schedule 0 auto, 1 wide, 2 ring)")
+                         "schedule 0 auto, 1 wide, 2 ring, 3 fine)")
     ap.add_argument("--rb-tiling", type=str, default="",
                     help="band:sweeps for the N=4096 red-black grid instead of autotuning (e.g. 0:5)")
     ap.add_argument("--rb-batch-band", type=str, default="",

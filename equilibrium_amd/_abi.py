@@ -19,7 +19,7 @@ DENSITY, SCRATCH, VX, VY, VX0, VY0 = range(6)
 DEFAULT_WALL, NO_WALL = 0, 1
 GS_WAVEFRONT, GS_REDBLACK = 0, 1
 BATCH_LARGEST_FIRST, BATCH_GIVEN_ORDER = 0, 1
-SCHED_AUTO, SCHED_WIDE, SCHED_RING = 0, 1, 2
+SCHED_AUTO, SCHED_WIDE, SCHED_RING, SCHED_FINE = 0, 1, 2, 3
 
 
 class EqfError(RuntimeError):

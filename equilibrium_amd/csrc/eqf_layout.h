@@ -113,10 +113,13 @@ struct SceneDesc {
 //   GS_RING: BS = 8, x0 handed on in LDS (a shorter stream lag that lets the
 //            rings fit, and fewer HBM bytes: large launches, bound by the
 //            memory system)
-enum GsSched { GS_WIDE = 0, GS_RING = 1 };
-constexpr int GS_D = 8;
+//   GS_FINE: BS = 4 and prefetch depth 4, x0 handed on in LDS (the shortest
+//            stream lag: the most sweep streams per strip for small scenes)
+enum GsSched { GS_WIDE = 0, GS_RING = 1, GS_FINE = 2 };
 constexpr int GS_B = 16;
-EQF_HD constexpr int gs_sync(int sched) { return sched == GS_RING ? 8 : 16; }
+EQF_HD constexpr int gs_sync(int sched) { return sched == GS_FINE ? 4 : sched == GS_RING ? 8 : 16; }
+EQF_HD constexpr int gs_depth(int sched) { return sched == GS_FINE ? 4 : 8; }
+EQF_HD constexpr bool gs_has_ring(int sched) { return sched != GS_WIDE; }
 
 EQF_HD constexpr int cyc_period(int N) { return (N + 15) & ~15; }
 
@@ -147,8 +150,8 @@ EQF_HD constexpr int cyc_stream_lag(int B, int BS, int D) {
 // Rows are P + 1 floats apart, so that lane 63's edge read (the strip below)
 // and the other lanes' shared read (the strip above) fall in different banks.
 EQF_HD constexpr int cyc_lds_row(int P) { return P + 1; }
-EQF_HD constexpr int gs_strip_lag(int sched) { return cyc_strip_lag(GS_B, gs_sync(sched), GS_D); }
-EQF_HD constexpr int gs_stream_lag(int sched) { return cyc_stream_lag(GS_B, gs_sync(sched), GS_D); }
+EQF_HD constexpr int gs_strip_lag(int sched) { return cyc_strip_lag(GS_B, gs_sync(sched), gs_depth(sched)); }
+EQF_HD constexpr int gs_stream_lag(int sched) { return cyc_stream_lag(GS_B, gs_sync(sched), gs_depth(sched)); }
 // Sweep streams per strip of a scene run by `waves` waves (lin_solves with
 // fewer sweeps than that use fewer).
 EQF_HD constexpr int cyc_streams(int waves, int ns, int P, int sched) {
@@ -173,9 +176,9 @@ EQF_HD constexpr int cyc_x0_ring(int B, int BS, int D) { return round_up(cyc_str
 // past them read x0 from HBM.
 constexpr int LDS_FLOATS_MAX = 160 * 1024 / 4;
 EQF_HD constexpr int cyc_x0_base(int ns, int P) { return (2 * ns + 1) * cyc_lds_row(P); }
-EQF_HD constexpr int cyc_x0_rings(int ns, int P, int m) {
-  const int R = cyc_x0_ring(GS_B, gs_sync(GS_RING), GS_D);
-  const int fit = (LDS_FLOATS_MAX - cyc_x0_base(ns, P) - GS_B * 64) / (R * 64);
+EQF_HD constexpr int cyc_x0_rings(int B, int BS, int D, int ns, int P, int m) {
+  const int R = cyc_x0_ring(B, BS, D);
+  const int fit = (LDS_FLOATS_MAX - cyc_x0_base(ns, P) - B * 64) / (R * 64);
   const int want = ns * (m - 1);
   return want < fit ? want : (fit > 0 ? fit : 0);
 }
@@ -183,15 +186,17 @@ EQF_HD constexpr int cyc_x0_rings(int ns, int P, int m) {
 // per row: 64 per strip).
 EQF_HD constexpr int cyc_lds_floats(int ns, int P, int waves, int sched) {
   int h = cyc_x0_base(ns, P);
-  if (sched == GS_RING)
-    h += GS_B * 64 + cyc_x0_rings(ns, P, cyc_streams(waves, ns, P, sched)) *
-                         cyc_x0_ring(GS_B, gs_sync(GS_RING), GS_D) * 64;
+  if (gs_has_ring(sched)) {
+    const int BS = gs_sync(sched), D = gs_depth(sched);
+    h += GS_B * 64 + cyc_x0_rings(GS_B, BS, D, ns, P, cyc_streams(waves, ns, P, sched)) *
+                         cyc_x0_ring(GS_B, BS, D) * 64;
+  }
   return h > 64 * ns ? h : 64 * ns;
 }
 // Every exact-mode launch (N up to 1024, 1..16 waves, either schedule) fits
 // the 160 KiB of LDS of a CU (N = P - 15 .. P share the strip count and period).
 constexpr bool cyc_lds_fits() {
-  for (int sched = GS_WIDE; sched <= GS_RING; ++sched)
+  for (int sched = GS_WIDE; sched <= GS_FINE; ++sched)
     for (int w = 1; w <= 16; ++w)
       for (int P = 16; P <= 1024; P += 16)
         if (cyc_lds_floats((P + 63) / 64, P, w, sched) > LDS_FLOATS_MAX) return false;

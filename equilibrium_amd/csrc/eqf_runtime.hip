@@ -638,6 +638,7 @@ static int batch_layout(eqf_batch* b) {
   }
   b->sched = b->sched_req == EQF_SCHED_RING   ? GS_RING
              : b->sched_req == EQF_SCHED_WIDE ? GS_WIDE
+             : b->sched_req == EQF_SCHED_FINE ? GS_FINE
                                               : eqf_pick_sched(maxns, bigP, b->max_waves);
   for (int i = 0; i < n; ++i) maxw = std::max(maxw, eqf_scene_waves(b->scenes[i]->ns, b->scenes[i]->P, b->sched));
   if (b->max_waves < maxns) return fail(EQF_EINVAL, "max_waves below the largest scene's strip count");
@@ -727,8 +728,8 @@ extern "C" int eqf_batch_set_band(eqf_batch* b, int tile, int sweeps_per_block) 
 extern "C" int eqf_batch_set_schedule(eqf_batch* b, int sched) {
   if (!b) return fail(EQF_EINVAL, "batch is NULL");
   if (b->rbg) return fail(EQF_EINVAL, "red-black batches have no wavefront schedule");
-  if (sched != EQF_SCHED_AUTO && sched != EQF_SCHED_WIDE && sched != EQF_SCHED_RING)
-    return fail(EQF_EINVAL, "sched must be EQF_SCHED_AUTO, EQF_SCHED_WIDE or EQF_SCHED_RING");
+  if (sched != EQF_SCHED_AUTO && sched != EQF_SCHED_WIDE && sched != EQF_SCHED_RING && sched != EQF_SCHED_FINE)
+    return fail(EQF_EINVAL, "sched must be EQF_SCHED_AUTO, EQF_SCHED_WIDE, EQF_SCHED_RING or EQF_SCHED_FINE");
   HIPCHK(hipSetDevice(b->ctx->device));
   HIPCHK(hipStreamSynchronize(b->stream));
   const int prev = b->sched_req;
@@ -771,45 +772,46 @@ extern "C" int eqf_batch_set_tuning(eqf_batch* b, int max_waves, int order) {
 hipError_t eqf_internal_launch(const SceneDesc* d_descs, int n, int W, int lds_floats, int nframes, int sched,
                                hipStream_t st) {
   const size_t lds = sizeof(float) * (size_t)lds_floats;
-  constexpr int SW = gs_sync(GS_WIDE), SR = gs_sync(GS_RING);
-  if (sched == GS_RING) {
-    if (W <= 8)
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SR, true, 512>), dim3(n), dim3(64 * W), lds, st, d_descs,
-                         nframes);
-    else
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SR, true, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
-                         nframes);
-  } else {
-    if (W <= 8)
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SW, false, 512>), dim3(n), dim3(64 * W), lds, st, d_descs,
-                         nframes);
-    else
-      hipLaunchKernelGGL((eqf_frames_cyclic<GS_D, GS_B, SW, false, 1024>), dim3(n), dim3(64 * W), lds, st, d_descs,
-                         nframes);
-  }
+#define EQF_LAUNCH_SCHED(S)                                                                               \
+  do {                                                                                                    \
+    if (W <= 8)                                                                                           \
+      hipLaunchKernelGGL((eqf_frames_cyclic<gs_depth(S), GS_B, gs_sync(S), gs_has_ring(S), 512>), dim3(n), \
+                         dim3(64 * W), lds, st, d_descs, nframes);                                        \
+    else                                                                                                  \
+      hipLaunchKernelGGL((eqf_frames_cyclic<gs_depth(S), GS_B, gs_sync(S), gs_has_ring(S), 1024>), dim3(n), \
+                         dim3(64 * W), lds, st, d_descs, nframes);                                        \
+  } while (0)
+  if (sched == GS_FINE) EQF_LAUNCH_SCHED(GS_FINE);
+  else if (sched == GS_RING) EQF_LAUNCH_SCHED(GS_RING);
+  else EQF_LAUNCH_SCHED(GS_WIDE);
+#undef EQF_LAUNCH_SCHED
   return hipGetLastError();
 }
 
 // Wavefront schedule of a launch whose largest scene (the longest frame
 // chain) has maxns strips and period P, with at most max_waves waves per
-// workgroup.  GS_RING hands x0 between sweep streams in LDS (fewer
-// vector-memory instructions and HBM bytes) and, with its shorter stream
-// lag, can run more streams per strip (P / 48 instead of P / 64), but needs
-// a barrier every 8 steps: it pays when it gives that scene more streams,
-// or two or more streams per strip and at least 8 waves; with the same one
-// or two streams and few waves per CU the barriers cost more than the ring
-// saves.  Measured (ms per frame, 20 sweeps, 16 waves at most, wide / ring,
-// one box): single scenes N=100 (1 / 2 streams) 1.72 / 1.11, N=128 (2 / 2
-// streams, 4 waves) 1.14 / 1.22, N=150 1.57 / 1.35, N=192 1.52 / 1.34, N=256
-// 2.11 / 2.02, N=320 3.20 / 3.00, N=448 6.02 / 5.65, N=512 7.37 / 6.98, N=600
-// (one stream) 12.97 / 13.38; batches: 32 N=512 7.78 / 7.37, 121 N=512
-// 11.02 / 10.00, the 512-scene mix 12.25 / 11.52, 130 N=256 2.88 / 2.40, 391
-// scenes N <= 256 2.95-3.18 / 2.95-2.98, 121 N=512 with 8 waves (one stream
-// per strip) 12.06 / 12.36.
+// workgroup.  The ring schedules hand x0 between sweep streams in LDS (fewer
+// vector-memory instructions and HBM bytes) and, with their shorter stream
+// lags, run more streams per strip (P / 48 for GS_RING, P / 32 for GS_FINE,
+// against P / 64), at the cost of a barrier every 8 or 4 steps.  The rule:
+// GS_FINE when it gives that scene more streams than GS_RING (N 49..192),
+// else GS_RING when it gives more than GS_WIDE or two or more streams and at
+// least 8 waves (N 193..512), else GS_WIDE (N > 512: one stream per strip;
+// N <= 48: one stream under every schedule).  Measured (ms per frame, 20 sweeps, 16 waves at
+// most, wide / ring / fine, one box): single scenes N=64 0.93 / 0.96 / 0.62,
+// N=100 1.73 / 1.12 / 0.92, N=128 1.14 / 1.22 / 0.86, N=150 1.56 / 1.34 /
+// 1.21, N=192 1.53 / 1.34 / 1.33, N=256 2.12 / 2.01 / 2.04, N=320 3.20 / 3.01
+// / 3.15, N=512 7.37 / 7.00 / 7.37, N=600 (one stream) 12.97 / 13.38 / -;
+// batches: 130 N=64 0.95 / 0.97 / 0.63, 118 N=128 1.18 / 1.27 / 0.91, 130
+// N=256 2.66 / 2.44 / 2.41, 391 scenes N <= 256 3.09 / 2.94 / 2.76, the
+// 512-scene mix 12.23 / 10.94 / 10.73 (the tuner picks per batch), 121 N=512
+// with 8 waves (one stream per strip) 12.06 / 12.36 / -.
 int eqf_pick_sched(int maxns, int P, int max_waves) {
-  const int m = cyc_streams(max_waves, maxns, P, GS_RING);
+  const int mf = cyc_streams(max_waves, maxns, P, GS_FINE);
+  const int mr = cyc_streams(max_waves, maxns, P, GS_RING);
   const int mw = cyc_streams(max_waves, maxns, P, GS_WIDE);
-  return (m > mw || (m >= 2 && maxns * m >= 8)) ? GS_RING : GS_WIDE;
+  if (mf > mr) return GS_FINE;
+  return (mr > mw || (mr >= 2 && maxns * mr >= 8)) ? GS_RING : GS_WIDE;
 }
 
 // Non-temporal streaming (SceneDesc::stream_l2) for the n scenes of a launch

@@ -238,7 +238,7 @@ __device__ __forceinline__ void gs_wavefront(const SceneDesc& sc, lf* __restrict
   constexpr int R = cyc_x0_ring(B, BS, D);
   static_assert(!XR || (R % BS == 0 && R >= LAG + BS && LAG - D >= BS && R >= B),
                 "x0 ring too short or read too early");
-  const int nrings = XR ? cyc_x0_rings(ns, P, m) : 0;
+  const int nrings = XR ? cyc_x0_rings(B, BS, D, ns, P, m) : 0;
   const bool rin = XR && has_strip && jst > 0 && s * (m - 1) + jst - 1 < nrings;
   const bool rout = XR && has_strip && jst < m - 1 && s * (m - 1) + jst < nrings;
   const int xsink = cyc_x0_base(ns, P);
@@ -892,12 +892,13 @@ __global__ void __launch_bounds__(MAXT) eqf_frames_cyclic(const eqf::SceneDesc* 
   for (int fr = 0; fr < nf; ++fr) eqf::frame_cyclic<D, B, BS, XR>(sc, (eqf::lf*)eqf_lds);
 }
 
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_WIDE), false, 1024>(
-    const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_WIDE), false, 512>(
-    const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_RING), true, 1024>(
-    const eqf::SceneDesc* __restrict__, int);
-template __global__ void eqf_frames_cyclic<eqf::GS_D, eqf::GS_B, eqf::gs_sync(eqf::GS_RING), true, 512>(
-    const eqf::SceneDesc* __restrict__, int);
+#define EQF_SCHED_KERNELS(S)                                                                              \
+  template __global__ void                                                                               \
+  eqf_frames_cyclic<eqf::gs_depth(S), eqf::GS_B, eqf::gs_sync(S), eqf::gs_has_ring(S), 1024>(           \
+      const eqf::SceneDesc* __restrict__, int);                                                          \
+  template __global__ void eqf_frames_cyclic<eqf::gs_depth(S), eqf::GS_B, eqf::gs_sync(S), eqf::gs_has_ring(S), 512>( \
+      const eqf::SceneDesc* __restrict__, int);
+EQF_SCHED_KERNELS(eqf::GS_WIDE)
+EQF_SCHED_KERNELS(eqf::GS_RING)
+EQF_SCHED_KERNELS(eqf::GS_FINE)
 

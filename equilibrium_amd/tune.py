@@ -78,7 +78,7 @@ def tune_redblack(scene, frames=1, use_cache=True):
 
 BATCH_WAVES = (16, 8, 4, 2, 1)
 BATCH_ORDERS = (_abi.BATCH_LARGEST_FIRST, _abi.BATCH_GIVEN_ORDER)
-BATCH_SCHEDS = (_abi.SCHED_WIDE, _abi.SCHED_RING)
+BATCH_SCHEDS = (_abi.SCHED_WIDE, _abi.SCHED_RING, _abi.SCHED_FINE)
 
 
 def _batch_key(scenes):

@@ -182,14 +182,15 @@ int eqf_batch_set_tuning(eqf_batch *b, int max_waves, int order);
  * barrier every 16 steps, every sweep stream reading a lin_solve's
  * right-hand side from device memory; EQF_SCHED_RING, a barrier every 8
  * steps, the right-hand side handed from sweep stream to sweep stream in
- * LDS (fewer device-memory reads) and, with a shorter lag between the
- * streams, up to P/48 streams per strip instead of P/64; EQF_SCHED_AUTO (the
- * default): RING when it gives the batch's largest scene more sweep streams
- * per strip, or two or more streams and at least 8 waves (N 81..112 and
- * 129..512 with max_waves 16), else WIDE.  Single scenes
- * (eqf_step) and server ticks use the AUTO rule.  Red-black batches:
+ * LDS and up to P/48 streams per strip instead of P/64; EQF_SCHED_FINE, a
+ * barrier every 4 steps, 4 steps of prefetch, the LDS hand-off and up to
+ * P/32 streams per strip (small scenes); EQF_SCHED_AUTO (the default): FINE
+ * when it gives the batch's largest scene more sweep streams than RING (N
+ * 49..192 with max_waves 16), else RING when it gives more than WIDE or two
+ * or more streams and at least 8 waves (N 193..512), else WIDE.  Single
+ * scenes (eqf_step) and server ticks use the AUTO rule.  Red-black batches:
  * EQF_EINVAL. */
-enum { EQF_SCHED_AUTO = 0, EQF_SCHED_WIDE = 1, EQF_SCHED_RING = 2 };
+enum { EQF_SCHED_AUTO = 0, EQF_SCHED_WIDE = 1, EQF_SCHED_RING = 2, EQF_SCHED_FINE = 3 };
 int eqf_batch_set_schedule(eqf_batch *b, int sched);
 /* Band shape and sweeps per launch of a red-black batch (as
  * eqf_scene_set_tuning); chosen by the autotuner. */

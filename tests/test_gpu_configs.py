@@ -96,7 +96,8 @@ def _mix_sample(per_size=16, frames=2):
 
 @pytest.mark.parametrize("max_waves,order,sched", [(8, _abi.BATCH_LARGEST_FIRST, _abi.SCHED_WIDE),
                                                    (16, _abi.BATCH_GIVEN_ORDER, _abi.SCHED_WIDE),
-                                                   (16, _abi.BATCH_LARGEST_FIRST, _abi.SCHED_RING)])
+                                                   (16, _abi.BATCH_LARGEST_FIRST, _abi.SCHED_RING),
+                                                   (16, _abi.BATCH_LARGEST_FIRST, _abi.SCHED_FINE)])
 def test_batched_mix_sample_bit_exact(max_waves, order, sched):
     sample = _mix_sample()
     assert len(sample) == 64 and sum(sc["size"] == 514 for sc in sample) == 16

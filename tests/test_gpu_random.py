@@ -61,7 +61,7 @@ def test_random_scenes_batched_equal_oracle():
         assert_state_equal(download_all(h), run_oracle(sc, frames=frames), f"batched random k={k + 24}")
 
 
-@pytest.mark.parametrize("sched", [_abi.SCHED_AUTO, _abi.SCHED_RING])
+@pytest.mark.parametrize("sched", [_abi.SCHED_AUTO, _abi.SCHED_RING, _abi.SCHED_FINE])
 def test_exact_batch_of_strip_boundary_sizes_equals_oracle(sched):
     """One fused launch over scenes at and around the strip (64 rows) and
     period (16 slots) boundaries, down to size 3, with sweep counts from 3 to

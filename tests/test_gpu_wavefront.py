@@ -268,7 +268,7 @@ def _noise_params(size):
     return p
 
 
-@pytest.mark.parametrize("sched", [_abi.SCHED_WIDE, _abi.SCHED_RING])
+@pytest.mark.parametrize("sched", [_abi.SCHED_WIDE, _abi.SCHED_RING, _abi.SCHED_FINE])
 def test_batch_tuning_keeps_results(sched):
     """Every launch shape the batch autotuner may pick (waves per workgroup,
     workgroup order, wavefront schedule) gives the oracle's bits.  With the
