@@ -165,10 +165,10 @@ static void batch_and_server_match_solo() {
     solo_d.push_back(d);
     detail::check(eqf_scene_destroy(h));
   }
-  // (b) one batch, under each wavefront schedule (N=298 runs 3 sweep streams
-  // per strip with the ring schedule)
-  std::vector<std::vector<float>> ring_d;
-  for (int sched : {EQF_SCHED_WIDE, EQF_SCHED_RING}) {
+  // (b) one batch, under each wavefront schedule (N=298 runs 2 / 3 / 3 sweep
+  // streams per strip under the wide / ring / fine schedule, N=98 1 / 2 / 3)
+  std::vector<std::vector<float>> ring_d, fine_d;
+  for (int sched : {EQF_SCHED_WIDE, EQF_SCHED_RING, EQF_SCHED_FINE}) {
     eqf_scene* hs[3];
     for (int i = 0; i < 3; ++i) hs[i] = make_scene(sizes[i], sd[i], su[i]);
     eqf_batch* b = nullptr;
@@ -183,7 +183,7 @@ static void batch_and_server_match_solo() {
     for (int i = 0; i < 3; ++i) {
       std::vector<float> d((size_t)sizes[i] * sizes[i]);
       detail::check(eqf_download(hs[i], EQF_DENSITY, d.data()));
-      (sched == EQF_SCHED_RING ? ring_d : batch_d).push_back(d);
+      (sched == EQF_SCHED_RING ? ring_d : sched == EQF_SCHED_FINE ? fine_d : batch_d).push_back(d);
       detail::check(eqf_scene_destroy(hs[i]));
     }
   }
@@ -227,6 +227,7 @@ static void batch_and_server_match_solo() {
   for (int i = 0; i < 3; ++i) {
     CHECK(std::memcmp(solo_d[i].data(), batch_d[i].data(), solo_d[i].size() * 4) == 0);
     CHECK(std::memcmp(solo_d[i].data(), ring_d[i].data(), solo_d[i].size() * 4) == 0);
+    CHECK(std::memcmp(solo_d[i].data(), fine_d[i].data(), solo_d[i].size() * 4) == 0);
     CHECK(std::memcmp(solo_d[i].data(), server_d[i].data(), solo_d[i].size() * 4) == 0);
   }
 }
