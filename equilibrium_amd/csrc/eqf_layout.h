@@ -104,7 +104,7 @@ struct SceneDesc {
 };
 
 // Exact-mode wavefront schedules (compile-time per kernel, so that every
-// access offset is an immediate): prefetch depth GS_D steps, blocks of
+// access offset is an immediate): prefetch depth gs_depth steps, blocks of
 // GS_B steps (per-block set-up, one selector word), a workgroup barrier every
 // BS steps (BS divides GS_B), and whether x0 goes from sweep stream to sweep
 // stream through LDS (cyc_x0_ring).
